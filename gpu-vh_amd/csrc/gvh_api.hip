@@ -1,0 +1,526 @@
+// gvh_api.hip -- C ABI of the KT step: context, device state, stepping.
+//
+// Device state of one lattice (the module globals of the reference's
+// DynamicalVariables.cu, made a context): three conserved buffers (q^n, the
+// predictor qS, the corrector Q), three velocity buffers (u^n, u^{n-1}, u*)
+// and two (e, p) buffers.  Buffers are var-major blocks of G doubles, G the
+// ghosted lattice size, in the reference's column-major cell order
+// s = i + ncx (j + ncy k) (DynamicalVariables.cu:33-35), so host transfers
+// are plain copies.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpuvh.h"
+#include "gvh_device.h"
+#include "gvh_internal.h"
+
+namespace gvh {
+hipError_t launch_stage_fp64(const StageArgs &A, int eos, bool bulk, bool stage2, hipStream_t st);
+hipError_t launch_count_above(const double *e, int nx, int ny, int nz, int ncx, int ncy, double thr,
+                              unsigned long long *out, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t st);
+}  // namespace gvh
+
+using namespace gvh;
+
+static thread_local std::string g_err;
+
+int gvh_fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" const char *gvh_last_error(void) { return g_err.c_str(); }
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return gvh_fail(GVH_EHIP, "%s failed: %s", #expr, hipGetErrorString(_e));  \
+  } while (0)
+
+namespace gvh {
+// ------------------------------------------------------------- host EoS
+// Host evaluation of equilibriumEnergyDensity (EquationOfState.cu:151-204)
+// for the freeze-out and regulation thresholds; the tabulated form inverts
+// the monotone T(e) table.
+static double node_e(const HostEos &H, int k) {
+  return std::ldexp(1.0 + (double)(k % H.M) / H.M, H.emin_exp + k / H.M);
+}
+
+double host_e_of_T(const HostEos &H, double T) {
+  if (H.kind == EOS_CONFORMAL) return EOS_FACTOR * std::pow(T, 4.0);
+  if (H.kind == EOS_TABLE) {
+    const int N = H.n;
+    const double *Tt = H.T.data();
+    if (T <= Tt[0]) return T <= 0 ? 0.0 : node_e(H, 0) * std::pow(T / Tt[0], 4.0);
+    if (T >= Tt[N - 1]) return node_e(H, N - 1) * std::pow(T / Tt[N - 1], 4.0);
+    int lo = 0, hi = N - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (Tt[mid] <= T) lo = mid; else hi = mid;
+    }
+    const double f = (T - Tt[lo]) / (Tt[hi] - Tt[lo]);
+    return node_e(H, lo) + f * (node_e(H, hi) - node_e(H, lo));
+  }
+  static const double num[24] = {
+      -0.011958188410851651, 119.89423098138208, -3156.9475699248055, 32732.86844374939,
+      -187899.8994764422, 712537.3610845465, -1.557049803609345e6, 1.4852519861308339e6,
+      532132.6079941876, -1.963099445042592e6, -4484.44579242679, 1.7984228830058286e6,
+      119345.25619517374, -1.3499773937058165e6, -207838.4995663606, 654970.2138652403,
+      -78643.00334616247, 40274.00078068926, 422619.58977657766, -409688.07836393174,
+      -62005.75915066359, 46788.14270090656, 40784.330477857235, -12589.47744840392};
+  static const double den[24] = {
+      31630.074365558292, -127100.88940643385, 173528.1225422275, -39403.297956865215,
+      -85582.57873541754, 9320.560804233442, 50882.74198960172, 20335.926473421183,
+      -14897.725710713818, -23836.484117457, -13726.013896090335, 4517.908673107615,
+      18056.19917986404, 14954.82860467155, 2569.623976952738, -9304.046211514986,
+      -15606.429173842751, 8383.710735812094, 1591.3177623932843, -678.748230997762,
+      -33.58687934953277, 3.2520554133126285, -0.19647288043440464, 0.005443394551264717};
+  double a = num[0], b = den[0], Tk = 1.0;
+  for (int k = 1; k < 24; ++k) {
+    Tk *= T;
+    a += num[k] * Tk;
+    b += den[k] * Tk;
+  }
+  return a / b;
+}
+
+// ---------------------------------------------------------- context
+
+static int check_dims(const gvh_lattice_parameters *L) {
+  if (!L) return gvh_fail(GVH_EINVAL, "lattice parameters are NULL");
+  if (L->numLatticePointsX < 1 || L->numLatticePointsY < 1 || L->numLatticePointsRapidity < 1)
+    return gvh_fail(GVH_EINVAL, "lattice dimensions must be positive (%d x %d x %d)",
+                    L->numLatticePointsX, L->numLatticePointsY, L->numLatticePointsRapidity);
+  const double G = (double)(L->numLatticePointsX + 4) * (L->numLatticePointsY + 4) *
+                   (L->numLatticePointsRapidity + 4);
+  if (G > 2.0e9) return gvh_fail(GVH_EINVAL, "lattice too large for 32-bit cell ids");
+  if (!(L->latticeSpacingX > 0 && L->latticeSpacingY > 0 && L->latticeSpacingRapidity > 0 &&
+        L->latticeSpacingProperTime > 0))
+    return gvh_fail(GVH_EINVAL, "lattice spacings must be positive");
+  return GVH_OK;
+}
+
+int eos_from_options(const gvh_options *o, HostEos &H) {
+  H.kind = o ? o->eos : EOS_WB;
+  if (H.kind == EOS_TABLE) {
+    if (!o->tab_p || !o->tab_cs2 || !o->tab_T || o->tab_n < 2 || o->tab_per_octave < 1)
+      return gvh_fail(GVH_EINVAL, "tabulated EoS needs tab_p, tab_cs2, tab_T, tab_n>=2");
+    H.n = o->tab_n;
+    H.M = o->tab_per_octave;
+    H.emin_exp = o->tab_emin_exp;
+    H.p.assign(o->tab_p, o->tab_p + H.n);
+    H.cs2.assign(o->tab_cs2, o->tab_cs2 + H.n);
+    H.T.assign(o->tab_T, o->tab_T + H.n);
+    for (int k = 1; k < H.n; ++k)
+      if (!(H.T[k] > H.T[k - 1])) return gvh_fail(GVH_EINVAL, "EoS table T(e) not increasing at %d", k);
+  } else if (H.kind != EOS_CONFORMAL && H.kind != EOS_WB) {
+    return gvh_fail(GVH_EINVAL, "unknown EoS kind %d", H.kind);
+  }
+  return GVH_OK;
+}
+
+int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T, hipStream_t st) {
+  T = EosTable{};
+  if (H.kind != EOS_TABLE) return GVH_OK;
+  const int N = H.n;
+  std::vector<double> pc(4 * (size_t)N), tt(2 * (size_t)N);
+  for (int k = 0; k < N; ++k) {
+    const int k1 = k + 1 < N ? k + 1 : k;
+    pc[4 * k + 0] = H.p[k];
+    pc[4 * k + 1] = H.p[k1] - H.p[k];
+    pc[4 * k + 2] = H.cs2[k];
+    pc[4 * k + 3] = H.cs2[k1] - H.cs2[k];
+    tt[2 * k + 0] = H.T[k];
+    tt[2 * k + 1] = H.T[k1] - H.T[k];
+  }
+  HIP_TRY(hipMalloc(dev_pc, pc.size() * sizeof(double)));
+  HIP_TRY(hipMalloc(dev_tt, tt.size() * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(*dev_pc, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(*dev_tt, tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  T.pc = (const double4 *)*dev_pc;
+  T.tt = (const double2 *)*dev_tt;
+  T.n = N;
+  T.M = H.M;
+  T.emin_exp = H.emin_exp;
+  T.e_first = node_e(H, 0);
+  T.e_second = node_e(H, 1);
+  T.e_pen = node_e(H, N - 2);
+  T.e_last = node_e(H, N - 1);
+  return GVH_OK;
+}
+
+}  // namespace gvh
+
+extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro_parameters *hydro,
+                          const gvh_options *opts, gvh_context **out) {
+  if (!out) return gvh_fail(GVH_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_dims(lattice);
+  if (rc) return rc;
+  if (!hydro) return gvh_fail(GVH_EINVAL, "hydro parameters are NULL");
+  if (!(hydro->shearViscosityToEntropyDensity > 0))
+    return gvh_fail(GVH_EINVAL, "eta/s must be positive");
+  if (opts && opts->precision != GVH_FP64 && opts->precision != GVH_MIXED)
+    return gvh_fail(GVH_EINVAL, "unknown precision %d", opts->precision);
+
+  gvh_context *c = new gvh_context();
+  c->L = *lattice;
+  c->Hp = *hydro;
+  c->nx = lattice->numLatticePointsX;
+  c->ny = lattice->numLatticePointsY;
+  c->nz = lattice->numLatticePointsRapidity;
+  c->ncx = c->nx + 4;
+  c->ncy = c->ny + 4;
+  c->ncz = c->nz + 4;
+  c->G = (long long)c->ncx * c->ncy * c->ncz;
+  c->bulk = opts ? (opts->bulk != 0) : true;
+  c->precision = opts ? opts->precision : GVH_FP64;
+  c->zeta_tpeak = (opts && opts->zeta_tpeak > 0) ? opts->zeta_tpeak : 1.01355;
+  rc = eos_from_options(opts, c->eos);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  if (opts && opts->stream) {
+    c->stream = (hipStream_t)opts->stream;
+    c->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+      delete c;
+      return gvh_fail(GVH_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  // one allocation for all state: 3 x 15 conserved, 3 x 4 velocity, 2 x 2 (e, p)
+  const size_t nbuf = 3 * 15 + 3 * 4 + 4;
+  const size_t bytes = nbuf * (size_t)c->G * sizeof(double);
+  hipError_t e = hipMalloc(&c->arena, bytes);
+  if (e != hipSuccess) {
+    gvh_destroy(c);
+    return gvh_fail(GVH_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(c->arena, 0, bytes, c->stream);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->diag, DIAG_N * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->count, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemsetAsync(c->diag, 0, DIAG_N * sizeof(unsigned long long), c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev[2]);
+  if (e != hipSuccess) {
+    gvh_destroy(c);
+    return gvh_fail(GVH_EHIP, "context set-up: %s", hipGetErrorString(e));
+  }
+  double *a = (double *)c->arena;
+  for (int b = 0; b < 3; ++b) c->qbuf[b] = a + (size_t)b * 15 * c->G;
+  a += 45 * (size_t)c->G;
+  for (int b = 0; b < 3; ++b) c->ubuf[b] = a + (size_t)b * 4 * c->G;
+  a += 12 * (size_t)c->G;
+  for (int b = 0; b < 2; ++b) {
+    c->ebuf[b] = a + (size_t)(2 * b) * c->G;
+    c->pbuf[b] = a + (size_t)(2 * b + 1) * c->G;
+  }
+  rc = upload_eos_table(c->eos, &c->tab_pc, &c->tab_tt, c->tab, c->stream);
+  if (rc) {
+    gvh_destroy(c);
+    return rc;
+  }
+  const double hbarc = 0.197326938;  // HydroPlugin.cpp:132
+  c->e_reg = host_e_of_T(c->eos, hydro->regulationTemperatureGeV / hbarc);
+  c->iq = 0;
+  c->iu = 0;
+  c->iup = 1;
+  *out = c;
+  return GVH_OK;
+}
+
+extern "C" int gvh_destroy(gvh_context *c) {
+  if (!c) return GVH_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 3; ++k)
+    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->diag) (void)hipFree(c->diag);
+  if (c->count) (void)hipFree(c->count);
+  if (c->tab_pc) (void)hipFree(c->tab_pc);
+  if (c->tab_tt) (void)hipFree(c->tab_tt);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GVH_OK;
+}
+
+// ghost fill on the host copy of the initial state (GhostCells.cu:66-144)
+static void host_ghosts(std::vector<double *> &arrs, int nx, int ny, int nz, int ncx, int ncy) {
+  auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)ncx * (j + (size_t)ncy * k); };
+  for (double *a : arrs) {
+    for (int k = 2; k < nz + 2; ++k)
+      for (int j = 2; j < ny + 2; ++j) {
+        a[id(0, j, k)] = a[id(1, j, k)] = a[id(2, j, k)];
+        a[id(nx + 2, j, k)] = a[id(nx + 3, j, k)] = a[id(nx + 1, j, k)];
+      }
+    for (int k = 2; k < nz + 2; ++k)
+      for (int i = 2; i < nx + 2; ++i) {
+        a[id(i, 0, k)] = a[id(i, 1, k)] = a[id(i, 2, k)];
+        a[id(i, ny + 2, k)] = a[id(i, ny + 3, k)] = a[id(i, ny + 1, k)];
+      }
+    for (int j = 2; j < ny + 2; ++j)
+      for (int i = 2; i < nx + 2; ++i) {
+        a[id(i, j, 0)] = a[id(i, j, 1)] = a[id(i, j, 2)];
+        a[id(i, j, nz + 2)] = a[id(i, j, nz + 3)] = a[id(i, j, nz + 1)];
+      }
+  }
+}
+
+extern "C" int gvh_set_initial_state(gvh_context *c, double t0, const double *e, const double *p,
+                                     const double *ut, const double *ux, const double *uy,
+                                     const double *un, const double *pi10, const double *Pi) {
+  (void)t0;
+  if (!c || !e || !p || !ut || !ux || !uy || !un)
+    return gvh_fail(GVH_EINVAL, "gvh_set_initial_state: NULL argument");
+  const int nx = c->nx, ny = c->ny, nz = c->nz, ncx = c->ncx, ncy = c->ncy;
+  const size_t G = (size_t)c->G, nint = (size_t)nx * ny * nz;
+  // host image of one conserved buffer, e, p and u (zero outside the interior)
+  std::vector<double> hq(15 * G, 0.0), he(G, 0.0), hp(G, 0.0), hu(4 * G, 0.0), hup(4 * G, 0.0);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t a = i + (size_t)nx * (j + (size_t)ny * k);
+        const size_t s = (i + 2) + (size_t)ncx * ((j + 2) + (size_t)ncy * (k + 2));
+        const double Pis = (Pi && c->bulk) ? Pi[a] : 0.0;
+        he[s] = e[a];
+        hp[s] = p[a];
+        hu[s] = ut[a];
+        hu[G + s] = ux[a];
+        hu[2 * G + s] = uy[a];
+        hu[3 * G + s] = un[a];
+        for (int m = 0; m < 10; ++m) hq[(4 + m) * G + s] = pi10 ? pi10[m * nint + a] : 0.0;
+        if (c->bulk) hq[14 * G + s] = Pis;
+        // setConservedVariables with Ttt..Ttn (EnergyMomentumTensor.cu:159-177)
+        const double P = p[a] + Pis;
+        hq[0 * G + s] = (e[a] + P) * ut[a] * ut[a] - P + hq[4 * G + s];
+        hq[1 * G + s] = (e[a] + P) * ut[a] * ux[a] + hq[5 * G + s];
+        hq[2 * G + s] = (e[a] + P) * ut[a] * uy[a] + hq[6 * G + s];
+        hq[3 * G + s] = (e[a] + P) * ut[a] * un[a] + hq[7 * G + s];
+      }
+  hup = hu;  // the previous-step velocity has zero ghosts (DynamicalVariables.cu:280-284)
+  std::vector<double *> arrs;
+  for (int n = 0; n < 15; ++n) arrs.push_back(hq.data() + n * G);
+  arrs.push_back(he.data());
+  arrs.push_back(hp.data());
+  for (int m = 0; m < 4; ++m) arrs.push_back(hu.data() + m * G);
+  host_ghosts(arrs, nx, ny, nz, ncx, ncy);
+
+  c->iq = 0;
+  c->iu = 0;
+  c->iup = 1;
+  c->ie = 0;
+  const size_t B = sizeof(double);
+  HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq], hq.data(), 15 * G * B, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ebuf[c->ie], he.data(), G * B, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->pbuf[c->ie], hp.data(), G * B, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ubuf[c->iu], hu.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ubuf[c->iup], hup.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->initialized = true;
+  return GVH_OK;
+}
+
+static double *field_ptr(gvh_context *c, int f) {
+  if (f >= GVH_TTT && f <= GVH_PI) return c->qbuf[c->iq] + (size_t)f * c->G;
+  if (f == GVH_E) return c->ebuf[c->ie];
+  if (f == GVH_P) return c->pbuf[c->ie];
+  if (f >= GVH_UT && f <= GVH_UN) return c->ubuf[c->iu] + (size_t)(f - GVH_UT) * c->G;
+  if (f >= GVH_UTP && f <= GVH_UNP) return c->ubuf[c->iup] + (size_t)(f - GVH_UTP) * c->G;
+  return nullptr;
+}
+
+extern "C" int gvh_set_field(gvh_context *c, int field, const double *host_ghosted) {
+  if (!c || !host_ghosted) return gvh_fail(GVH_EINVAL, "gvh_set_field: NULL argument");
+  double *d = field_ptr(c, field);
+  if (!d) return gvh_fail(GVH_EINVAL, "unknown field %d", field);
+  HIP_TRY(hipMemcpyAsync(d, host_ghosted, (size_t)c->G * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GVH_OK;
+}
+
+extern "C" int gvh_get_field(gvh_context *c, int field, int ghosted, double *host_out) {
+  if (!c || !host_out) return gvh_fail(GVH_EINVAL, "gvh_get_field: NULL argument");
+  double *d = field_ptr(c, field);
+  if (!d) return gvh_fail(GVH_EINVAL, "unknown field %d", field);
+  if (ghosted) {
+    HIP_TRY(hipMemcpyAsync(host_out, d, (size_t)c->G * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GVH_OK;
+  }
+  // interior rows: one 2-D copy per eta slice
+  for (int k = 0; k < c->nz; ++k) {
+    const double *src = d + 2 + (size_t)c->ncx * (2 + (size_t)c->ncy * (k + 2));
+    double *dst = host_out + (size_t)c->nx * c->ny * k;
+    HIP_TRY(hipMemcpy2DAsync(dst, c->nx * sizeof(double), src, c->ncx * sizeof(double),
+                             c->nx * sizeof(double), c->ny, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GVH_OK;
+}
+
+// Arguments of one stage launch for the current buffer roles.
+static StageArgs stage_args(gvh_context *c, int stage, double t) {
+  StageArgs A{};
+  const int ie_in = stage == 1 ? c->ie : 1 - c->ie;
+  const int q_in = stage == 1 ? c->iq : c->iqs();
+  A.qin = c->qbuf[q_in];
+  A.qn = c->qbuf[c->iq];
+  A.qout = c->qbuf[stage == 1 ? c->iqs() : c->iQ()];
+  A.uin = c->ubuf[stage == 1 ? c->iu : c->ius()];
+  A.uprev = c->ubuf[stage == 1 ? c->iup : c->iu];
+  A.uout = c->ubuf[stage == 1 ? c->ius() : c->iup];
+  A.ein = c->ebuf[ie_in];
+  A.pin = c->pbuf[ie_in];
+  A.eout = c->ebuf[1 - ie_in];
+  A.pout = c->pbuf[1 - ie_in];
+  A.diag = c->diag;
+  A.G = c->G;
+  A.nx = c->nx;
+  A.ny = c->ny;
+  A.nz = c->nz;
+  A.ncx = c->ncx;
+  A.ncy = c->ncy;
+  const double dt = c->L.latticeSpacingProperTime;
+  A.t = stage == 1 ? t : t + dt;
+  A.tinf = t + dt;
+  A.dt = dt;
+  A.inv_dt = 1.0 / dt;
+  const double h[3] = {c->L.latticeSpacingX, c->L.latticeSpacingY, c->L.latticeSpacingRapidity};
+  for (int d = 0; d < 3; ++d) {
+    A.h[d] = h[d];
+    A.inv_h[d] = 1.0 / h[d];
+    A.half_inv_h[d] = 1.0 / h[d] / 2;  // 1 / d_dx / 2, SourceTerms.cu:449
+  }
+  A.etabar = c->Hp.shearViscosityToEntropyDensity;
+  A.zeta_tpeak = c->zeta_tpeak;
+  A.e_reg = c->e_reg;
+  A.tab = c->tab;
+  return A;
+}
+
+int launch_stage(gvh_context *c, const StageArgs &A, bool stage2);
+
+// Buffer rotation of one step: stage 1 writes qS, e/p[1-ie], u*; stage 2
+// writes Q, e/p[ie], and u^{n+1} into the u^{n-1} buffer; then the roles
+// swap like setCurrentConservedVariables and swapFluidVelocity.
+static void rotate_after_step(gvh_context *c) {
+  const int old_u = c->iu;
+  c->iu = c->iup;   // u^{n+1} was written into the u^{n-1} buffer
+  c->iup = old_u;   // u^n becomes the previous-step velocity
+  c->iq = c->iQ();  // q^{n+1} is in Q; the old q^n buffer becomes Q
+}
+
+static int step_impl(gvh_context *c, double t, float *ms1, float *ms2) {
+  if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  if (!c->initialized) return gvh_fail(GVH_EINVAL, "state not initialised (gvh_set_initial_state)");
+  const bool timed = ms1 || ms2;
+  if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  StageArgs A1 = stage_args(c, 1, t);
+  int rc = launch_stage(c, A1, false);
+  if (rc) return rc;
+  if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  StageArgs A2 = stage_args(c, 2, t);
+  rc = launch_stage(c, A2, true);
+  if (rc) return rc;
+  if (timed) {
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (ms1) *ms1 = a;
+    if (ms2) *ms2 = b;
+  }
+  rotate_after_step(c);
+  return GVH_OK;
+}
+
+int launch_stage(gvh_context *c, const StageArgs &A, bool stage2) {
+  hipError_t e = launch_stage_fp64(A, c->eos.kind, c->bulk, stage2, c->stream);
+  if (e != hipSuccess) return gvh_fail(GVH_EHIP, "stage kernel launch: %s", hipGetErrorString(e));
+  return GVH_OK;
+}
+
+extern "C" int gvh_step(gvh_context *c, double t) { return step_impl(c, t, nullptr, nullptr); }
+
+extern "C" int gvh_steps(gvh_context *c, double t0, int n_begin, int nsteps) {
+  if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  if (nsteps < 0) return gvh_fail(GVH_EINVAL, "nsteps must be >= 0");
+  const double dt = c->L.latticeSpacingProperTime;
+  for (int n = n_begin; n < n_begin + nsteps; ++n) {
+    int rc = step_impl(c, t0 + n * dt, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return GVH_OK;
+}
+
+extern "C" int gvh_step_timed(gvh_context *c, double t, float *ms1, float *ms2) {
+  float a = 0, b = 0;
+  int rc = step_impl(c, t, &a, &b);
+  if (ms1) *ms1 = a;
+  if (ms2) *ms2 = b;
+  return rc;
+}
+
+extern "C" int gvh_synchronize(gvh_context *c) {
+  if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GVH_OK;
+}
+
+extern "C" int gvh_count_above(gvh_context *c, double thr, long long *count) {
+  if (!c || !count) return gvh_fail(GVH_EINVAL, "gvh_count_above: NULL argument");
+  HIP_TRY(launch_fill_u64(c->count, 1, 0ull, c->stream));
+  HIP_TRY(launch_count_above(c->ebuf[c->ie], c->nx, c->ny, c->nz, c->ncx, c->ncy, thr, c->count,
+                             c->stream));
+  unsigned long long h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, c->count, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *count = (long long)h;
+  return GVH_OK;
+}
+
+extern "C" int gvh_get_diagnostics(gvh_context *c, gvh_diagnostics *out) {
+  if (!c || !out) return gvh_fail(GVH_EINVAL, "gvh_get_diagnostics: NULL argument");
+  unsigned long long h[DIAG_N] = {0};
+  HIP_TRY(hipMemcpyAsync(h, c->diag, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  out->newton_not_converged = h[DIAG_NOT_CONVERGED];
+  out->e_nan = h[DIAG_E_NAN];
+  out->e_clamped = h[DIAG_E_CLAMPED];
+  return GVH_OK;
+}
+
+extern "C" int gvh_energy_density_of_T(gvh_context *c, double T, double *e) {
+  if (!c || !e) return gvh_fail(GVH_EINVAL, "gvh_energy_density_of_T: NULL argument");
+  *e = host_e_of_T(c->eos, T);
+  return GVH_OK;
+}
+
+extern "C" int gvh_dims(gvh_context *c, int *nx, int *ny, int *nz) {
+  if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  if (nx) *nx = c->nx;
+  if (ny) *ny = c->ny;
+  if (nz) *nz = c->nz;
+  return GVH_OK;
+}

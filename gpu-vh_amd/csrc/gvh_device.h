@@ -1,0 +1,263 @@
+// gvh_device.h -- device math of the KT step for gfx950 (FP64 form).
+//
+// Reference files are cited per function (paths relative to rhic/src).
+// All state arithmetic is double.  Divisions that the reference performs by
+// a per-launch constant (dt, dx, t) are replaced by a multiplication with a
+// host-computed reciprocal; that, FMA contraction and the flux written as
+// (u_dir/u^tau)*q change results at the rounding level only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gvh {
+
+constexpr int NG = 2;             // ghost layers (LatticeParameters.h:13-15)
+constexpr int MAX_ITERS = 1000000; // EnergyMomentumTensor.cu:15
+constexpr double THETA = 2.0;     // FluxLimiter.cu:15
+constexpr double EOS_FACTOR = 13.8997; // EquationOfState.cuh:24
+
+enum { EOS_CONFORMAL = 0, EOS_WB = 1, EOS_TABLE = 2 };
+
+// counters of failure events, indexes into StageArgs::diag
+enum { DIAG_NOT_CONVERGED = 0, DIAG_E_NAN = 1, DIAG_E_CLAMPED = 2, DIAG_N = 4 };
+
+// Tabulated EoS on the device.  Node k sits at
+// e_k = 2^(emin_exp + k/M) (1 + (k % M)/M); per node the value and the
+// difference to node k+1 are stored so that an interpolation is one load.
+struct EosTable {
+  const double4 *pc;  // {p_k, p_{k+1}-p_k, cs2_k, cs2_{k+1}-cs2_k}
+  const double2 *tt;  // {T_k, T_{k+1}-T_k}
+  int n, M, emin_exp;
+  double e_first, e_second;  // e_0, e_1
+  double e_pen, e_last;      // e_{n-2}, e_{n-1}
+};
+
+struct StageArgs {
+  const double *__restrict__ qin;   // stage input conserved, 15 x G
+  const double *__restrict__ qn;    // q^n for the convex average (stage 2)
+  double *__restrict__ qout;        // stage output conserved
+  const double *__restrict__ uin;   // u of the stage input, 4 x G
+  const double *__restrict__ uprev; // u used for the time derivative
+  double *__restrict__ uout;
+  const double *__restrict__ ein;
+  const double *__restrict__ pin;
+  double *__restrict__ eout;
+  double *__restrict__ pout;
+  unsigned long long *__restrict__ diag;
+  long long G;          // stride between variables
+  int nx, ny, nz, ncx, ncy;
+  double t;             // time of the Euler step
+  double tinf;          // time of the inferred-variable solve
+  double dt, inv_dt;
+  double h[3], inv_h[3], half_inv_h[3];  // dx, dy, deta and reciprocals
+  double etabar;        // eta/s
+  double zeta_tpeak;    // zeta/s temperature scale
+  double e_reg;         // regulation below this energy density
+  EosTable tab;
+};
+
+// ------------------------------------------------------------ limiter
+// sign / minmod / approximateDerivative, FluxLimiter.cu:17-39; sgn(0)=+1
+__device__ __forceinline__ double sgn(double x) { return x < 0.0 ? -1.0 : 1.0; }
+__device__ __forceinline__ double minmod(double x, double y) {
+  return (sgn(x) + sgn(y)) * fmin(fabs(x), fabs(y)) * 0.5;
+}
+__device__ __forceinline__ double slope(double x, double y, double z) {
+  return minmod(THETA * (y - x), minmod((z - x) * 0.5, THETA * (z - y)));
+}
+
+// ---------------------------------------------------------------- EoS
+struct TabPos {
+  int k;
+  double fr;
+};
+
+__device__ __forceinline__ TabPos tab_locate(const EosTable &T, double e) {
+  TabPos r;
+  if (e >= T.e_first && e < T.e_last) {
+    int E;
+    const double m = frexp(e, &E);  // e = m 2^E, m in [0.5, 1)
+    const double f = (2.0 * m - 1.0) * (double)T.M;
+    const int j = (int)f;
+    r.fr = f - (double)j;
+    r.k = (E - 1 - T.emin_exp) * T.M + j;
+    if (r.k > T.n - 2) {  // f rounded up onto the last node
+      r.k = T.n - 2;
+      r.fr += 1.0;
+    }
+  } else if (e >= T.e_last) {  // beyond the table, +inf included
+    r.k = T.n - 2;
+    r.fr = (e - T.e_pen) / (T.e_last - T.e_pen);
+  } else {  // below the first node, or NaN
+    r.k = 0;
+    r.fr = (e - T.e_first) / (T.e_second - T.e_first);
+  }
+  return r;
+}
+
+// equilibriumPressure (EquationOfState.cu:31-80) and speedOfSoundSquared
+// (:83-116); the conformal c_s^2 is the reference's integer 1/3 == 0.
+template <int EOS>
+__device__ __forceinline__ void eos_p_cs2(const EosTable &T, double e, double &p,
+                                          double &cs2) {
+  if constexpr (EOS == EOS_CONFORMAL) {
+    p = e / 3.0;
+    cs2 = 0.0;
+  } else if constexpr (EOS == EOS_TABLE) {
+    TabPos q = tab_locate(T, e);
+    double4 v = T.pc[q.k];
+    p = v.x + q.fr * v.y;
+    cs2 = v.z + q.fr * v.w;
+  } else {
+    double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e;
+    double e7 = e6 * e, e8 = e7 * e, e9 = e8 * e, e10 = e9 * e, e11 = e10 * e;
+    double e12 = e11 * e, e13 = e12 * e;
+    double a = fma(1.9531729608963267e-11, e12, fma(3.1188455176941583e-7, e11,
+               fma(0.0009417586777847889, e10, fma(0.7158279081255019, e9,
+               fma(141.5073484468774, e8, fma(6340.448389300905, e7,
+               fma(41913.439282708554, e6, fma(334334.4309240126, e5,
+               fma(1.6357487344679043e6, e4, fma(3.1729694865420084e6, e3,
+               fma(1.077580993288114e6, e2, fma(9737.845799644809, e,
+               -0.25181736420168666))))))))))));
+    double b = fma(3.2581066229887368e-18, e12, fma(5.928138360995685e-11, e11,
+               fma(9.601103399348206e-7, e10, fma(0.002962497695527404, e9,
+               fma(2.3405487982094204, e8, fma(499.04919730607065, e7,
+               fma(26452.34905933697, e6, fma(278581.2989342773, e5,
+               fma(1.7851642641834426e6, e4, fma(1.3512402226067686e7, e3,
+               fma(2.0931169138134286e7, e2, fma(4.0574329080826794e6, e,
+               45829.44617893836))))))))))));
+    p = a / b;
+    double cn = 5.191934309650155e-32 + 4.123605749683891e-23 * e +
+                3.1955868410879504e-16 * e2 + 1.4170364808063119e-10 * e3 +
+                6.087136671592452e-6 * e4 + 0.02969737949090831 * e5 +
+                15.382615282179595 * e6 + 460.6487249985994 * e7 +
+                1612.4245252438795 * e8 + 275.0492627924299 * e9 +
+                58.60283714484669 * e10 + 6.504847576502024 * e11 +
+                0.03009027913262399 * e12 + 8.189430244031285e-6 * e13;
+    double cd = 1.4637868900982493e-30 + 6.716598285341542e-22 * e +
+                3.5477700458515908e-15 * e2 + 1.1225580509306008e-9 * e3 +
+                0.00003551782901018317 * e4 + 0.13653226327408863 * e5 +
+                60.85769171450653 * e6 + 1800.5461219450308 * e7 +
+                15190.225535036281 * e8 + 590.2572000057821 * e9 +
+                293.99144775704605 * e10 + 21.461303090563028 * e11 +
+                0.09301685073435291 * e12 + 0.000024810902623582917 * e13;
+    cs2 = cn / cd;
+  }
+}
+
+template <int EOS>
+__device__ __forceinline__ double eos_p(const EosTable &T, double e) {
+  if constexpr (EOS == EOS_CONFORMAL) {
+    return e / 3.0;
+  } else if constexpr (EOS == EOS_TABLE) {
+    TabPos q = tab_locate(T, e);
+    double4 v = T.pc[q.k];
+    return v.x + q.fr * v.y;
+  } else {
+    double p, c;
+    eos_p_cs2<EOS>(T, e, p, c);
+    return p;
+  }
+}
+
+// effectiveTemperature, EquationOfState.cu:119-148
+template <int EOS>
+__device__ __forceinline__ double eos_T(const EosTable &T, double e) {
+  if constexpr (EOS == EOS_CONFORMAL) {
+    return pow(e / EOS_FACTOR, 0.25);
+  } else if constexpr (EOS == EOS_TABLE) {
+    TabPos q = tab_locate(T, e);
+    double2 v = T.tt[q.k];
+    return v.x + q.fr * v.y;
+  } else {
+    double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e;
+    double e7 = e6 * e, e8 = e7 * e, e9 = e8 * e, e10 = e9 * e, e11 = e10 * e;
+    double n = 1.510073201405604e-29 + 8.014062800678687e-18 * e +
+               2.4954778310451065e-10 * e2 + 0.000063810382643387 * e3 +
+               0.4873490574161924 * e4 + 207.48582344326206 * e5 +
+               6686.07424325115 * e6 + 14109.766109389702 * e7 +
+               1471.6180520527757 * e8 + 14.055788949565482 * e9 +
+               0.015421252394182246 * e10 + 1.5780479034557783e-6 * e11;
+    double d = 7.558667139355393e-28 + 1.3686372302041508e-16 * e +
+               2.998130743142826e-9 * e2 + 0.0005036835870305458 * e3 +
+               2.316902328874072 * e4 + 578.0778724946719 * e5 +
+               11179.193315394154 * e6 + 17965.67607192861 * e7 +
+               1051.0730543534657 * e8 + 5.916312075925817 * e9 +
+               0.003778342768228011 * e10 + 1.8472801679382593e-7 * e11;
+    return n / d;
+  }
+}
+
+// bulkViscosityToEntropyDensity, TransportCoefficients.cu:27-36
+__device__ __forceinline__ double zeta_over_s(double T, double tpeak) {
+  double x = T / tpeak;
+  if (x > 1.05) return 0.9 * exp(-(x - 1) / 0.025) + 0.25 * exp(-(x - 1) / 0.13) + 0.001;
+  if (x < 0.995) return 0.9 * exp((x - 1) / 0.0025) + 0.22 * exp((x - 1) / 0.022) + 0.03;
+  return -13.77 * x * x + 27.55 * x - 13.45;
+}
+
+// ------------------------------------------------------------ root-find
+// energyDensityFromConservedVariables, EnergyMomentumTensor.cu:19-48:
+// Newton on e with the caller's guess, stopping at |de| <= 1e-3 |e|.
+template <int EOS>
+__device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, double M0,
+                                          double M, double Pi,
+                                          unsigned long long *diag) {
+  if constexpr (EOS == EOS_CONFORMAL) {
+    return fabs(sqrt(fabs(4.0 * M0 * M0 - 3.0 * M)) - M0);
+  } else {
+    double e0 = ePrev;
+    for (int it = 0; it < MAX_ITERS; ++it) {
+      double p, cs2;
+      eos_p_cs2<EOS>(T, e0, p, cs2);
+      double cst2 = p / e0;
+      double A = fma(M0, 1.0 - cst2, Pi);
+      double B = fma(M0, M0 + Pi, -M);
+      double H = sqrt(fabs(A * A + 4.0 * cst2 * B));
+      double D = (A - H) / (2.0 * cst2);
+      double f = e0 + D;
+      double dc = cs2 - cst2;
+      double fp = 1.0 - (dc * (B + D * H - (dc * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
+      double e = (it < MAX_ITERS / 2) ? e0 - f / fp : e0 - 0.9 * f / fp;
+      if (fabs(e - e0) <= 0.001 * fabs(e)) return e;
+      e0 = e;
+    }
+    atomicAdd(&diag[DIAG_NOT_CONVERGED], 1ull);
+    return e0;
+  }
+}
+
+struct Prim {
+  double e, p, ut, ux, uy, un;
+};
+
+// getInferredVariables, EnergyMomentumTensor.cu:51-108 (the '#ifdef Pi'
+// consistency fix at :82-85 is never compiled in the reference; omitted)
+template <int EOS>
+__device__ __forceinline__ Prim inferred(const EosTable &T, double t, double ttt,
+                                         double ttx, double tty, double ttn,
+                                         double pitt, double pitx, double pity,
+                                         double pitn, double Pi, double ePrev,
+                                         unsigned long long *diag) {
+  double M0 = ttt - pitt, M1 = ttx - pitx, M2 = tty - pity, M3 = ttn - pitn;
+  double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+  Prim r;
+  r.e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag);
+  if (isnan(r.e)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
+  r.p = eos_p<EOS>(T, r.e);
+  if (r.e < 1.e-7) {
+    r.e = 1.e-7;
+    r.p = 1.e-7;
+    atomicAdd(&diag[DIAG_E_CLAMPED], 1ull);
+  }
+  double P = r.p + Pi;
+  double E = 1.0 / (r.e + P);
+  r.ut = sqrt(fabs((M0 + P) * E));
+  double E2 = E / r.ut;
+  r.ux = M1 * E2;
+  r.uy = M2 * E2;
+  r.un = M3 * E2;
+  return r;
+}
+
+}  // namespace gvh

@@ -1,0 +1,62 @@
+// gvh_internal.h -- context layout shared by the library's host sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <vector>
+
+#include "../../include/gpuvh.h"
+#include "gvh_device.h"
+
+namespace gvh {
+
+// Host copy of the equation of state of a context.
+struct HostEos {
+  int kind = EOS_WB;
+  int n = 0, M = 1, emin_exp = 0;
+  std::vector<double> p, cs2, T;
+};
+
+double host_e_of_T(const HostEos &H, double T);
+int eos_from_options(const gvh_options *o, HostEos &H);
+int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T, hipStream_t st);
+
+}  // namespace gvh
+
+int gvh_fail(int code, const char *fmt, ...);
+
+struct gvh_context {
+  gvh_lattice_parameters L{};
+  gvh_hydro_parameters Hp{};
+  int nx = 0, ny = 0, nz = 0, ncx = 0, ncy = 0, ncz = 0;
+  long long G = 0;
+  bool bulk = true;
+  int precision = GVH_FP64;
+  double zeta_tpeak = 1.01355;
+  double e_reg = 0.0;
+  gvh::HostEos eos;
+  gvh::EosTable tab{};
+  void *tab_pc = nullptr, *tab_tt = nullptr;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+  void *arena = nullptr;
+  double *qbuf[3] = {nullptr, nullptr, nullptr};  // [0],[1]: q^n / Q (swap), [2]: qS
+  double *ubuf[3] = {nullptr, nullptr, nullptr};  // u^n, u^{n-1}, u* (rotate)
+  double *ebuf[2] = {nullptr, nullptr};
+  double *pbuf[2] = {nullptr, nullptr};
+  unsigned long long *diag = nullptr;
+  unsigned long long *count = nullptr;
+
+  int iq = 0;   // buffer holding q^n
+  int iu = 0;   // buffer holding u^n
+  int iup = 1;  // buffer holding u^{n-1}
+  int ie = 0;   // (e, p) buffer holding the current state
+  bool initialized = false;
+
+  int iQ() const { return 1 - iq; }
+  int iqs() const { return 2; }
+  int ius() const { return 3 - iu - iup; }
+};

@@ -1,0 +1,304 @@
+"""Python host binding of the MI355X gpu-vh KT library (libgpuvh.so).
+
+The library is the product: a C ABI (include/gpuvh.h) over HIP kernels for
+gfx950.  This module is a thin ctypes layer mirroring the reference's driver
+surface -- LatticeParameters / HydroParameters / InitialConditionParameters,
+the conserved-variable field order and twoStepRungeKutta -- so that tests,
+benchmarks and embedding code can drive it.  It never falls back to a CPU
+implementation: if the shared library is missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libgpuvh.so")
+DATA_DIR = os.path.join(PKG_DIR, "data")
+EOS_TABLE_PATH = os.path.join(DATA_DIR, "eos_qcdlike_2000.dat")
+
+HBARC = 0.197326938  # GeV fm (rhic/src/HydroPlugin.cpp:132)
+
+EOS_CONFORMAL, EOS_WB, EOS_TABLE = 0, 1, 2
+FP64, MIXED = 0, 1
+
+# field ids, include/gpuvh.h
+(TTT, TTX, TTY, TTN, PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, PIYN, PINN, PI,
+ E, P, UT, UX, UY, UN, UTP, UXP, UYP, UNP) = range(25)
+FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy",
+               "pixn", "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
+               "utp", "uxp", "uyp", "unp"]
+
+
+class GvhError(RuntimeError):
+    pass
+
+
+class LatticeParameters(C.Structure):
+    """rhic/include/LatticeParameters.h:17-31"""
+    _fields_ = [
+        ("numLatticePointsX", C.c_int), ("numLatticePointsY", C.c_int),
+        ("numLatticePointsRapidity", C.c_int),
+        ("numComputationalLatticePointsX", C.c_int), ("numComputationalLatticePointsY", C.c_int),
+        ("numComputationalLatticePointsRapidity", C.c_int),
+        ("numProperTimePoints", C.c_int),
+        ("latticeSpacingX", C.c_double), ("latticeSpacingY", C.c_double),
+        ("latticeSpacingRapidity", C.c_double), ("latticeSpacingProperTime", C.c_double),
+    ]
+
+
+class InitialConditionParameters(C.Structure):
+    """rhic/include/InitialConditionParameters.h:13-26"""
+    _fields_ = [
+        ("initialConditionType", C.c_int), ("numberOfNucleonsPerNuclei", C.c_int),
+        ("numberOfSourceFiles", C.c_int), ("initialEnergyDensity", C.c_double),
+        ("scatteringCrossSectionNN", C.c_double), ("impactParameter", C.c_double),
+        ("fractionOfBinaryCollisions", C.c_double), ("rapidityVariance", C.c_double),
+        ("rapidityMean", C.c_double), ("sourceType", C.c_int),
+    ]
+
+
+class HydroParameters(C.Structure):
+    """rhic/include/HydroParameters.h:15-22"""
+    _fields_ = [
+        ("initialProperTimePoint", C.c_double), ("shearViscosityToEntropyDensity", C.c_double),
+        ("freezeoutTemperatureGeV", C.c_double), ("initializePimunuNavierStokes", C.c_int),
+        ("regulationTemperatureGeV", C.c_double),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("eos", C.c_int), ("bulk", C.c_int), ("precision", C.c_int),
+        ("zeta_tpeak", C.c_double),
+        ("tab_n", C.c_int), ("tab_per_octave", C.c_int), ("tab_emin_exp", C.c_int),
+        ("tab_p", C.POINTER(C.c_double)), ("tab_cs2", C.POINTER(C.c_double)),
+        ("tab_T", C.POINTER(C.c_double)),
+        ("stream", C.c_void_p),
+    ]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [("newton_not_converged", C.c_ulonglong), ("e_nan", C.c_ulonglong),
+                ("e_clamped", C.c_ulonglong)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgpuvh.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GvhError("libgpuvh.so not built (%s); run __graft_entry__.build() or "
+                       "make -C gpu-vh_amd" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    L.gvh_last_error.restype = C.c_char_p
+    sig = {
+        "gvh_read_lattice_parameters": [C.c_char_p, C.POINTER(LatticeParameters)],
+        "gvh_read_hydro_parameters": [C.c_char_p, C.POINTER(HydroParameters)],
+        "gvh_read_ic_parameters": [C.c_char_p, C.POINTER(InitialConditionParameters)],
+        "gvh_create": [C.POINTER(LatticeParameters), C.POINTER(HydroParameters),
+                       C.POINTER(Options), C.POINTER(C.c_void_p)],
+        "gvh_destroy": [C.c_void_p],
+        "gvh_set_initial_state": [C.c_void_p, C.c_double] + [dp] * 8,
+        "gvh_set_field": [C.c_void_p, C.c_int, dp],
+        "gvh_step": [C.c_void_p, C.c_double],
+        "gvh_steps": [C.c_void_p, C.c_double, C.c_int, C.c_int],
+        "gvh_step_timed": [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "gvh_synchronize": [C.c_void_p],
+        "gvh_get_field": [C.c_void_p, C.c_int, C.c_int, dp],
+        "gvh_count_above": [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)],
+        "gvh_get_diagnostics": [C.c_void_p, C.POINTER(Diagnostics)],
+        "gvh_energy_density_of_T": [C.c_void_p, C.c_double, dp],
+        "gvh_dims": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().gvh_last_error()
+        raise GvhError("gpuvh error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def read_lattice_parameters(config_dir):
+    L = LatticeParameters()
+    _check(lib().gvh_read_lattice_parameters(config_dir.encode(), C.byref(L)))
+    return L
+
+
+def read_hydro_parameters(config_dir):
+    H = HydroParameters()
+    _check(lib().gvh_read_hydro_parameters(config_dir.encode(), C.byref(H)))
+    return H
+
+
+def read_ic_parameters(config_dir):
+    I = InitialConditionParameters()
+    _check(lib().gvh_read_ic_parameters(config_dir.encode(), C.byref(I)))
+    return I
+
+
+def lattice(nx, ny, nz, dx, dy, dz, dt, nt=0):
+    L = LatticeParameters()
+    L.numLatticePointsX, L.numLatticePointsY, L.numLatticePointsRapidity = nx, ny, nz
+    L.numComputationalLatticePointsX = nx + 4
+    L.numComputationalLatticePointsY = ny + 4
+    L.numComputationalLatticePointsRapidity = nz + 4
+    L.numProperTimePoints = nt
+    L.latticeSpacingX, L.latticeSpacingY = dx, dy
+    L.latticeSpacingRapidity, L.latticeSpacingProperTime = dz, dt
+    return L
+
+
+def hydro(t0, eta_over_s, T_fo_GeV=0.15, T_reg_GeV=0.0):
+    H = HydroParameters()
+    H.initialProperTimePoint = t0
+    H.shearViscosityToEntropyDensity = eta_over_s
+    H.freezeoutTemperatureGeV = T_fo_GeV
+    H.initializePimunuNavierStokes = 0
+    H.regulationTemperatureGeV = T_reg_GeV
+    return H
+
+
+class EosTable:
+    """Tabulated EoS on the pseudo-log grid (see data/make_eos_table.py)."""
+
+    def __init__(self, path=EOS_TABLE_PATH):
+        n = per_octave = emin_exp = None
+        with open(path) as f:
+            for line in f:
+                if not line.startswith("#"):
+                    break
+                tok = line[1:].split()
+                if tok and tok[0] == "nodes":
+                    n, per_octave, emin_exp = int(tok[1]), int(tok[3]), int(tok[5])
+        d = np.loadtxt(path)
+        self.e, self.p, self.cs2, self.T = (np.ascontiguousarray(c) for c in d.T)
+        self.n = n if n is not None else len(self.e)
+        self.per_octave, self.emin_exp = per_octave, emin_exp
+
+    def node_e(self, k):
+        k = np.asarray(k)
+        return np.ldexp(1.0 + (k % self.per_octave) / self.per_octave,
+                        self.emin_exp + k // self.per_octave)
+
+
+def options(eos=EOS_WB, bulk=True, precision=FP64, zeta_tpeak=1.01355, table=None,
+            stream=None):
+    o = Options()
+    o.eos, o.bulk, o.precision, o.zeta_tpeak = eos, int(bulk), precision, zeta_tpeak
+    o._keep = []
+    if eos == EOS_TABLE:
+        if table is None:
+            table = EosTable()
+        o.tab_n, o.tab_per_octave, o.tab_emin_exp = table.n, table.per_octave, table.emin_exp
+        for name in ("p", "cs2", "T"):
+            a = np.ascontiguousarray(getattr(table, name), dtype=np.float64)
+            o._keep.append(a)
+            setattr(o, "tab_" + name, _ptr(a))
+    o.stream = stream
+    return o
+
+
+class Context:
+    """One lattice's device state (the reference's module globals as an object)."""
+
+    def __init__(self, lattice_params, hydro_params, opts):
+        self.L, self.H, self.opts = lattice_params, hydro_params, opts
+        h = C.c_void_p()
+        _check(lib().gvh_create(C.byref(lattice_params), C.byref(hydro_params),
+                                C.byref(opts), C.byref(h)))
+        self.h = h
+        self.nx = lattice_params.numLatticePointsX
+        self.ny = lattice_params.numLatticePointsY
+        self.nz = lattice_params.numLatticePointsRapidity
+        self.dt = lattice_params.latticeSpacingProperTime
+
+    def close(self):
+        if self.h:
+            lib().gvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _flat(a):
+        return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+    def set_initial_state(self, t0, e, p, ut, ux, uy, un, pi10=None, Pi=None):
+        arrs = [self._flat(a) for a in (e, p, ut, ux, uy, un)]
+        pi = None if pi10 is None else np.ascontiguousarray(
+            np.stack([self._flat(a) for a in pi10]))
+        Pi_a = None if Pi is None else self._flat(Pi)
+        _check(lib().gvh_set_initial_state(self.h, t0, *[_ptr(a) for a in arrs],
+                                           _ptr(pi), _ptr(Pi_a)))
+
+    def step(self, t):
+        _check(lib().gvh_step(self.h, t))
+
+    def steps(self, t0, n_begin, nsteps):
+        _check(lib().gvh_steps(self.h, t0, n_begin, nsteps))
+
+    def step_timed(self, t):
+        a, b = C.c_float(), C.c_float()
+        _check(lib().gvh_step_timed(self.h, t, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self):
+        _check(lib().gvh_synchronize(self.h))
+
+    def get(self, field, ghosted=False):
+        """Field as an (nx, ny, nz) array (x fastest) or the ghosted grid."""
+        if ghosted:
+            shape = (self.nx + 4, self.ny + 4, self.nz + 4)
+        else:
+            shape = (self.nx, self.ny, self.nz)
+        out = np.empty(int(np.prod(shape)), dtype=np.float64)
+        _check(lib().gvh_get_field(self.h, field, int(ghosted), _ptr(out)))
+        return out.reshape(shape, order="F")
+
+    def set_field(self, field, ghosted_array):
+        a = self._flat(ghosted_array)
+        _check(lib().gvh_set_field(self.h, field, _ptr(a)))
+
+    def count_above(self, e_threshold):
+        c = C.c_longlong()
+        _check(lib().gvh_count_above(self.h, e_threshold, C.byref(c)))
+        return c.value
+
+    def diagnostics(self):
+        d = Diagnostics()
+        _check(lib().gvh_get_diagnostics(self.h, C.byref(d)))
+        return {"newton_not_converged": d.newton_not_converged, "e_nan": d.e_nan,
+                "e_clamped": d.e_clamped}
+
+    def energy_density_of_T(self, T):
+        e = C.c_double()
+        _check(lib().gvh_energy_density_of_T(self.h, T, C.byref(e)))
+        return e.value

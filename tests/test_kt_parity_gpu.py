@@ -1,0 +1,122 @@
+"""Per-cell parity of the HIP KT step against the CPU FP64 oracle.
+
+Bar (FP64 form): after the stated number of steps every cell of e, u^mu and
+pi^{mu nu} (and Pi when evolved) agrees with the oracle to 1e-10 relative,
+with a floor of 1e-6 x the field's maximum for components that cross zero.
+"""
+import numpy as np
+import pytest
+
+import gpuvh
+import oracle_lib as O
+from gpuvh import ic
+from parity import PI_FIELDS, U_FIELDS, Case, compare, hotspot_state, rel_err, run_both
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _check(errs, tol=TOL):
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, "relative errors above %g: %s (all: %s)" % (tol, bad, errs)
+
+
+def test_c1_conformal_shear_200_steps():
+    """61x61x1, dx=0.1, tau0=0.5, dtau=0.01, 200 steps, p=e/3, eta/s=0.2,
+    Gaussian + 10 seeded hotspots, u at rest, pi=0 (seed 1)."""
+    case = Case(61, 61, 1, 0.1, 0.1, 0.01, 0.5, 0.2, bulk=False, eos=gpuvh.EOS_CONFORMAL)
+    e = ic.c1_event(seed=1)
+    ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+    state = (e, e / 3.0, ut, ux, uy, un, pi10, Pi)
+    orc, ctx = run_both(case, state, 200)
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS)
+    print("C1 errors", errs)
+    _check(errs)
+    assert ctx.diagnostics()["e_nan"] == 0
+
+
+@pytest.mark.parametrize("eos", [gpuvh.EOS_WB, gpuvh.EOS_TABLE])
+def test_3d_shear_bulk_small(eos):
+    """21x21x11 3+1D event with shear and bulk, 20 steps, both non-conformal
+    equations of state (Newton root-find path)."""
+    zeta = 0.16 / gpuvh.HBARC if eos == gpuvh.EOS_TABLE else 1.01355
+    case = Case(21, 21, 11, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=eos, zeta_tpeak=zeta)
+    state = hotspot_state(case, seed=7, n_hot=12, amp=6.0)
+    orc, ctx = run_both(case, state, 20)
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    print("3D errors", eos, errs)
+    _check(errs)
+
+
+def test_regulation_active_with_flow():
+    """Regulation below T_reg = 0.25 GeV with random pi and a flowing initial
+    state (exercises regulateDissipativeCurrents' rescaling)."""
+    case = Case(17, 15, 5, 0.1, 0.1, 0.01, 0.6, 0.2, bulk=True, eos=gpuvh.EOS_WB,
+                T_reg_GeV=0.25)
+    state = hotspot_state(case, seed=11, n_hot=8, amp=3.0, pi_noise=0.05, u_noise=0.05)
+    orc, ctx = run_both(case, state, 10)
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    print("regulation errors", errs)
+    _check(errs)
+    # the rescaling must actually have happened somewhere
+    e = orc.get(O.E)
+    assert np.any(e < orc_e_reg(case)), "no cell below the regulation energy density"
+
+
+def orc_e_reg(case):
+    return O.lib().kto_e_of_T(case.oracle().P, case.T_reg_GeV / gpuvh.HBARC)
+
+
+def test_2d_table_eos_with_flow_and_pi():
+    """2+1D, tabulated EoS, bulk, flowing state with initial pi: 31x27x1,
+    30 steps (ragged nx != ny)."""
+    case = Case(31, 27, 1, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=gpuvh.EOS_TABLE,
+                zeta_tpeak=0.16 / gpuvh.HBARC)
+    state = hotspot_state(case, seed=3, n_hot=10, amp=6.0, pi_noise=0.02, u_noise=0.1)
+    orc, ctx = run_both(case, state, 30)
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    print("2D table errors", errs)
+    _check(errs)
+
+
+def test_ghost_cells_match_oracle():
+    """Full ghosted grid (faces) after 3 steps matches the oracle's ghost fill."""
+    case = Case(9, 7, 5, 0.1, 0.1, 0.01, 0.5, 0.2, bulk=True, eos=gpuvh.EOS_WB)
+    state = hotspot_state(case, seed=5, n_hot=4, amp=4.0, u_noise=0.05)
+    orc, ctx = run_both(case, state, 3)
+    for f in (gpuvh.E, gpuvh.TTT, gpuvh.PIXX, gpuvh.UX):
+        a = ctx.get(f, ghosted=True)
+        b = orc.get(f, ghosted=True)
+        # edges/corners are never written by either side; compare face cells
+        nx, ny, nz = case.nx, case.ny, case.nz
+        m = np.zeros(a.shape, bool)
+        m[:, 2:ny + 2, 2:nz + 2] = True
+        m[2:nx + 2, :, 2:nz + 2] = True
+        m[2:nx + 2, 2:ny + 2, :] = True
+        assert rel_err(a[m], b[m]) <= TOL, gpuvh.FIELD_NAMES[f]
+
+
+def test_count_above_matches_numpy():
+    case = Case(21, 19, 7, 0.1, 0.1, 0.01, 0.5, 0.2, bulk=False, eos=gpuvh.EOS_CONFORMAL)
+    state = hotspot_state(case, seed=9)
+    ctx = case.context()
+    ctx.set_initial_state(case.t0, *state)
+    e = ctx.get(gpuvh.E)
+    for thr in (0.0, float(np.median(e)), float(e.max()), float(e.max()) + 1.0):
+        assert ctx.count_above(thr) == int(np.sum(e > thr))
+
+
+def test_field_roundtrip_and_errors():
+    case = Case(5, 6, 3, 0.1, 0.1, 0.01, 0.5, 0.2, bulk=True, eos=gpuvh.EOS_WB)
+    ctx = case.context()
+    with pytest.raises(gpuvh.GvhError):
+        ctx.step(0.5)  # no initial state yet
+    state = hotspot_state(case, seed=2, n_hot=3)
+    ctx.set_initial_state(case.t0, *state)
+    g = np.arange((5 + 4) * (6 + 4) * (3 + 4), dtype=np.float64).reshape((9, 10, 7), order="F")
+    ctx.set_field(gpuvh.PIXY, g)
+    assert np.array_equal(ctx.get(gpuvh.PIXY, ghosted=True), g)
+    assert np.array_equal(ctx.get(gpuvh.PIXY), g[2:7, 2:8, 2:5])
+    with pytest.raises(gpuvh.GvhError):
+        ctx.get(99)
