@@ -96,6 +96,57 @@ double host_e_of_T(const HostEos &H, double T) {
   return a / b;
 }
 
+// Host evaluation of equilibriumPressure (EquationOfState.cu:31-80) with the
+// same table lookup as the device (initial pressures, diagnostics).
+double host_eos_p(const HostEos &H, double e) {
+  if (H.kind == EOS_CONFORMAL) return e / 3.0;
+  if (H.kind == EOS_TABLE) {
+    const int N = H.n, M = H.M;
+    const double e_first = node_e(H, 0), e_last = node_e(H, N - 1);
+    int k;
+    double fr;
+    if (e >= e_first && e < e_last) {
+      int E;
+      const double m = std::frexp(e, &E);
+      const double f = (2.0 * m - 1.0) * (double)M;
+      const int j = (int)f;
+      fr = f - (double)j;
+      k = (E - 1 - H.emin_exp) * M + j;
+      if (k > N - 2) {
+        k = N - 2;
+        fr += 1.0;
+      }
+    } else if (e >= e_last) {
+      k = N - 2;
+      fr = (e - node_e(H, N - 2)) / (e_last - node_e(H, N - 2));
+    } else {
+      k = 0;
+      fr = (e - e_first) / (node_e(H, 1) - e_first);
+    }
+    return H.p[k] + fr * (H.p[k + 1] - H.p[k]);
+  }
+  static const double a[13] = {-0.25181736420168666, 9737.845799644809, 1.077580993288114e6,
+                               3.1729694865420084e6, 1.6357487344679043e6, 334334.4309240126,
+                               41913.439282708554, 6340.448389300905, 141.5073484468774,
+                               0.7158279081255019, 0.0009417586777847889, 3.1188455176941583e-7,
+                               1.9531729608963267e-11};
+  static const double b[13] = {45829.44617893836, 4.0574329080826794e6, 2.0931169138134286e7,
+                               1.3512402226067686e7, 1.7851642641834426e6, 278581.2989342773,
+                               26452.34905933697, 499.04919730607065, 2.3405487982094204,
+                               0.002962497695527404, 9.601103399348206e-7, 5.928138360995685e-11,
+                               3.2581066229887368e-18};
+  double ek[13];
+  ek[0] = 1.0;
+  ek[1] = e;
+  for (int i = 2; i < 13; ++i) ek[i] = ek[i - 1] * e;
+  double na = std::fma(a[1], ek[1], a[0]), nb = std::fma(b[1], ek[1], b[0]);
+  for (int i = 2; i < 13; ++i) {
+    na = std::fma(a[i], ek[i], na);
+    nb = std::fma(b[i], ek[i], nb);
+  }
+  return na / nb;
+}
+
 // ---------------------------------------------------------- context
 
 static int check_dims(const gvh_lattice_parameters *L) {
@@ -514,6 +565,12 @@ extern "C" int gvh_get_diagnostics(gvh_context *c, gvh_diagnostics *out) {
 extern "C" int gvh_energy_density_of_T(gvh_context *c, double T, double *e) {
   if (!c || !e) return gvh_fail(GVH_EINVAL, "gvh_energy_density_of_T: NULL argument");
   *e = host_e_of_T(c->eos, T);
+  return GVH_OK;
+}
+
+extern "C" int gvh_equilibrium_pressure(gvh_context *c, long long n, const double *e, double *p) {
+  if (!c || (n > 0 && (!e || !p))) return gvh_fail(GVH_EINVAL, "gvh_equilibrium_pressure: NULL argument");
+  for (long long i = 0; i < n; ++i) p[i] = host_eos_p(c->eos, e[i]);
   return GVH_OK;
 }
 

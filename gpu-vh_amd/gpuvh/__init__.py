@@ -114,6 +114,7 @@ def lib():
         "gvh_count_above": [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)],
         "gvh_get_diagnostics": [C.c_void_p, C.POINTER(Diagnostics)],
         "gvh_energy_density_of_T": [C.c_void_p, C.c_double, dp],
+        "gvh_equilibrium_pressure": [C.c_void_p, C.c_longlong, dp, dp],
         "gvh_dims": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     }
     for name, args in sig.items():
@@ -297,6 +298,12 @@ class Context:
         _check(lib().gvh_get_diagnostics(self.h, C.byref(d)))
         return {"newton_not_converged": d.newton_not_converged, "e_nan": d.e_nan,
                 "e_clamped": d.e_clamped}
+
+    def equilibrium_pressure(self, e):
+        e = np.ascontiguousarray(np.asarray(e, dtype=np.float64))
+        p = np.empty_like(e)
+        _check(lib().gvh_equilibrium_pressure(self.h, e.size, _ptr(e), _ptr(p)))
+        return p
 
     def energy_density_of_T(self, T):
         e = C.c_double()

@@ -173,6 +173,10 @@ int gvh_get_diagnostics(gvh_context *ctx, gvh_diagnostics *out);
 /* Equation-of-state helpers of the context (host evaluation):
  * equilibriumEnergyDensity(T) and effectiveTemperature(e). */
 int gvh_energy_density_of_T(gvh_context *ctx, double T, double *e);
+/* equilibriumPressure (EquationOfState.cu:31-80) of n energy densities, for
+ * initial conditions: the reference sets p = p(e) when it reads e.dat
+ * (InitialConditions.cpp:127). */
+int gvh_equilibrium_pressure(gvh_context *ctx, long long n, const double *e, double *p);
 
 /* Lattice geometry of the context. */
 int gvh_dims(gvh_context *ctx, int *nx, int *ny, int *nz);

@@ -229,6 +229,11 @@ double kto_e_of_T(const kto_params *P, double T) {
   return num / den;
 }
 
+void kto_eos_p_batch(const kto_params *P, long n, const double *e, double *p) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) p[i] = eos_p(P, e[i]);
+}
+
 void kto_eos(const kto_params *P, double e, double *p, double *cs2, double *T) {
   *p = eos_p(P, e);
   *cs2 = eos_cs2(P, e);

@@ -78,6 +78,7 @@ double kto_minmod(double x, double y);
 double kto_approx_deriv(double x, double y, double z);
 void kto_eos(const kto_params *P, double e, double *p, double *cs2, double *T);
 double kto_e_of_T(const kto_params *P, double T);
+void kto_eos_p_batch(const kto_params *P, long n, const double *e, double *p);
 double kto_zeta_over_s(const kto_params *P, double T);
 /* q[15] -> out[6] = e, p, ut, ux, uy, un */
 void kto_inferred(const kto_params *P, double t, const double *q, double ePrev,

@@ -63,6 +63,7 @@ def lib():
         L.kto_approx_deriv.restype = C.c_double
         L.kto_approx_deriv.argtypes = [C.c_double] * 3
         L.kto_eos.argtypes = [C.POINTER(Params), C.c_double, dp, dp, dp]
+        L.kto_eos_p_batch.argtypes = [C.POINTER(Params), C.c_long, dp, dp]
         L.kto_e_of_T.restype = C.c_double
         L.kto_e_of_T.argtypes = [C.POINTER(Params), C.c_double]
         L.kto_zeta_over_s.restype = C.c_double
@@ -164,6 +165,13 @@ def eos(params, e):
     p, cs2, T = C.c_double(), C.c_double(), C.c_double()
     lib().kto_eos(C.byref(params), e, C.byref(p), C.byref(cs2), C.byref(T))
     return p.value, cs2.value, T.value
+
+
+def eos_p_batch(params, e):
+    e = np.ascontiguousarray(np.asarray(e, dtype=np.float64))
+    out = np.empty_like(e)
+    lib().kto_eos_p_batch(C.byref(params), e.size, _ptr(e), _ptr(out))
+    return out
 
 
 def inferred(params, t, q, e_prev):

@@ -12,16 +12,47 @@ PI_FIELDS = [gpuvh.PITT, gpuvh.PITX, gpuvh.PITY, gpuvh.PITN, gpuvh.PIXX, gpuvh.P
 U_FIELDS = [gpuvh.UT, gpuvh.UX, gpuvh.UY, gpuvh.UN]
 
 
-def rel_err(x, ref, floor_frac=1e-6):
-    """max_cells |x - ref| / max(|ref|, floor_frac * max_cells |ref|).
+def rel_err(x, ref, floor_frac=1e-6, scale=None):
+    """max_cells |x - ref| / max(|ref|, floor_frac * scale).
 
-    The floor keeps components that cross zero (u^x, pi^{xy}, ...) from
-    dividing by ~0; a field that is zero everywhere must match to 1e-300."""
-    scale = float(np.max(np.abs(ref)))
+    scale defaults to the field's own maximum; a field that is zero
+    everywhere must match exactly."""
+    if scale is None:
+        scale = float(np.max(np.abs(ref)))
     if scale == 0.0:
-        return float(np.max(np.abs(x))) / 1e-300 if np.any(x != 0) else 0.0
+        return 0.0 if not np.any(x != ref) else float("inf")
     den = np.maximum(np.abs(ref), floor_frac * scale)
     return float(np.max(np.abs(x - ref) / den))
+
+
+# number of eta indices of each field: in Milne coordinates a component with
+# n eta indices carries tau^-n relative to its orthonormal-frame value
+ETA_INDICES = {gpuvh.UN: 1, gpuvh.PITN: 1, gpuvh.PIXN: 1, gpuvh.PIYN: 1, gpuvh.PINN: 2}
+
+
+def floor_scales(ref, tau):
+    """Per-field scale for the relative-error floor: components of u^mu and
+    of pi^{mu nu} are floored at 1e-6 of the largest component of the same
+    tensor in the orthonormal frame (tau^n_eta x component), converted back
+    to the field's own units.  Components that vanish by symmetry (u^eta,
+    pi^{x eta}, ... on the eta = 0 plane) are computed as sums of O(1) terms
+    that cancel, so their rounding noise is set by the tensor's scale, not by
+    their own.  Scalars (e, Pi) use their own maximum."""
+    def phys_max(fields):
+        return max(float(np.max(np.abs(ref[f]))) * tau ** ETA_INDICES.get(f, 0) for f in fields)
+    out = {}
+    u_f = [f for f in U_FIELDS if f in ref]
+    pi_f = [f for f in PI_FIELDS if f in ref]
+    su = phys_max(u_f) if u_f else 0.0
+    sp = phys_max(pi_f) if pi_f else 0.0
+    for f in ref:
+        if f in U_FIELDS:
+            out[f] = su / tau ** ETA_INDICES.get(f, 0)
+        elif f in PI_FIELDS:
+            out[f] = sp / tau ** ETA_INDICES.get(f, 0)
+        else:
+            out[f] = float(np.max(np.abs(ref[f])))
+    return out
 
 
 class Case:
@@ -48,9 +79,7 @@ class Case:
         return gpuvh.Context(L, H, o)
 
     def pressure(self, e):
-        orc = self.oracle()
-        f = np.vectorize(lambda x: O.eos(orc.P, float(x))[0])
-        return f(e)
+        return O.eos_p_batch(self.oracle().P, e)
 
 
 def hotspot_state(case, seed, n_hot=20, amp=8.0, sigma=0.5, pi_noise=0.0, u_noise=0.0):
@@ -93,9 +122,20 @@ def run_both(case, state, nsteps, precision=gpuvh.FP64):
     return orc, ctx
 
 
-def compare(orc, ctx, fields):
-    """{field name: relative error} for the listed fields."""
+def compare(orc, ctx, fields, tau, report=None):
+    """{field name: relative error} for the listed fields, with the floors
+    of floor_scales.  report (a list) collects the worst cell per field."""
+    ref = {f: orc.get(f) for f in fields}
+    got = {f: ctx.get(f) for f in fields}
+    scales = floor_scales(ref, tau)
     out = {}
     for f in fields:
-        out[gpuvh.FIELD_NAMES[f]] = rel_err(ctx.get(f), orc.get(f))
+        name = gpuvh.FIELD_NAMES[f]
+        out[name] = rel_err(got[f], ref[f], scale=scales[f])
+        if report is not None and scales[f] > 0:
+            den = np.maximum(np.abs(ref[f]), 1e-6 * scales[f])
+            r = np.abs(got[f] - ref[f]) / den
+            w = np.unravel_index(np.argmax(r), r.shape)
+            report.append((name, out[name], w, float(ref[f][w]), float(got[f][w] - ref[f][w]),
+                           scales[f]))
     return out

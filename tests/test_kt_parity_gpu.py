@@ -1,8 +1,11 @@
 """Per-cell parity of the HIP KT step against the CPU FP64 oracle.
 
 Bar (FP64 form): after the stated number of steps every cell of e, u^mu and
-pi^{mu nu} (and Pi when evolved) agrees with the oracle to 1e-10 relative,
-with a floor of 1e-6 x the field's maximum for components that cross zero.
+pi^{mu nu} (and Pi when evolved) agrees with the oracle to 1e-10 relative.
+The denominator is floored at 1e-6 of the tensor's largest component in the
+orthonormal frame (tests/parity.py floor_scales): components that vanish by
+symmetry are sums of O(1) terms that cancel, so their rounding noise scales
+with the tensor, not with themselves.
 """
 import numpy as np
 import pytest
@@ -17,8 +20,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _check(errs, tol=TOL):
+def _check(errs, tol=TOL, report=None):
     bad = {k: v for k, v in errs.items() if not v <= tol}
+    if report:
+        for row in report:
+            print("  %-5s rel %.2e at %s ref %.3e diff %.2e scale %.3e" % row)
     assert not bad, "relative errors above %g: %s (all: %s)" % (tol, bad, errs)
 
 
@@ -30,9 +36,10 @@ def test_c1_conformal_shear_200_steps():
     ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
     state = (e, e / 3.0, ut, ux, uy, un, pi10, Pi)
     orc, ctx = run_both(case, state, 200)
-    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS)
+    rep = []
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS, tau=0.5 + 200 * 0.01, report=rep)
     print("C1 errors", errs)
-    _check(errs)
+    _check(errs, report=rep)
     assert ctx.diagnostics()["e_nan"] == 0
 
 
@@ -44,9 +51,11 @@ def test_3d_shear_bulk_small(eos):
     case = Case(21, 21, 11, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=eos, zeta_tpeak=zeta)
     state = hotspot_state(case, seed=7, n_hot=12, amp=6.0)
     orc, ctx = run_both(case, state, 20)
-    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    rep = []
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI], tau=0.5 + 20 * 0.01,
+                   report=rep)
     print("3D errors", eos, errs)
-    _check(errs)
+    _check(errs, report=rep)
 
 
 def test_regulation_active_with_flow():
@@ -56,9 +65,11 @@ def test_regulation_active_with_flow():
                 T_reg_GeV=0.25)
     state = hotspot_state(case, seed=11, n_hot=8, amp=3.0, pi_noise=0.05, u_noise=0.05)
     orc, ctx = run_both(case, state, 10)
-    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    rep = []
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI], tau=0.6 + 10 * 0.01,
+                   report=rep)
     print("regulation errors", errs)
-    _check(errs)
+    _check(errs, report=rep)
     # the rescaling must actually have happened somewhere
     e = orc.get(O.E)
     assert np.any(e < orc_e_reg(case)), "no cell below the regulation energy density"
@@ -75,9 +86,11 @@ def test_2d_table_eos_with_flow_and_pi():
                 zeta_tpeak=0.16 / gpuvh.HBARC)
     state = hotspot_state(case, seed=3, n_hot=10, amp=6.0, pi_noise=0.02, u_noise=0.1)
     orc, ctx = run_both(case, state, 30)
-    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI])
+    rep = []
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI], tau=0.5 + 30 * 0.01,
+                   report=rep)
     print("2D table errors", errs)
-    _check(errs)
+    _check(errs, report=rep)
 
 
 def test_ghost_cells_match_oracle():
