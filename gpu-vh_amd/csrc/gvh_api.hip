@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -210,6 +211,8 @@ int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T
   T.e_second = node_e(H, 1);
   T.e_pen = node_e(H, N - 2);
   T.e_last = node_e(H, N - 1);
+  T.inv_first_span = 1.0 / (T.e_second - T.e_first);
+  T.inv_last_span = 1.0 / (T.e_last - T.e_pen);
   return GVH_OK;
 }
 
@@ -294,6 +297,12 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
   c->iq = 0;
   c->iu = 0;
   c->iup = 1;
+  // kernel arrangement and traversal strip height; GVH_VARIANT / GVH_STRIP
+  // override them for tuning sweeps
+  if (const char *v = getenv("GVH_VARIANT")) c->variant = atoi(v);
+  if (const char *v = getenv("GVH_STRIP")) c->strip = atoi(v);
+  if (c->strip < 1) c->strip = 1;
+  if (c->strip > c->ny) c->strip = c->ny;
   *out = c;
   return GVH_OK;
 }
@@ -466,6 +475,9 @@ static StageArgs stage_args(gvh_context *c, int stage, double t) {
   A.zeta_tpeak = c->zeta_tpeak;
   A.e_reg = c->e_reg;
   A.tab = c->tab;
+  A.strip = c->strip;
+  A.nstrips = (c->ny + c->strip - 1) / c->strip;
+  A.variant = c->variant;
   return A;
 }
 

@@ -30,6 +30,8 @@ struct EosTable {
   int n, M, emin_exp;
   double e_first, e_second;  // e_0, e_1
   double e_pen, e_last;      // e_{n-2}, e_{n-1}
+  double inv_first_span;     // 1/(e_1 - e_0)
+  double inv_last_span;      // 1/(e_{n-1} - e_{n-2})
 };
 
 struct StageArgs {
@@ -54,16 +56,54 @@ struct StageArgs {
   double zeta_tpeak;    // zeta/s temperature scale
   double e_reg;         // regulation below this energy density
   EosTable tab;
+  int strip, nstrips;   // y rows per traversal strip, number of strips
+  int variant;          // kernel arrangement of a stage (gvh_stage.hip)
 };
 
 // ------------------------------------------------------------ limiter
-// sign / minmod / approximateDerivative, FluxLimiter.cu:17-39; sgn(0)=+1
-__device__ __forceinline__ double sgn(double x) { return x < 0.0 ? -1.0 : 1.0; }
-__device__ __forceinline__ double minmod(double x, double y) {
-  return (sgn(x) + sgn(y)) * fmin(fabs(x), fabs(y)) * 0.5;
-}
+// approximateDerivative (FluxLimiter.cu:33-39): the nested generalized
+// minmod(theta*dl, minmod(dc, theta*dr)) with sgn(0)=+1 equals
+// sign * min(|theta dl|, |dc|, |theta dr|) when all three arguments are in the
+// same sign class (x < 0 versus x >= 0) and +0 otherwise -- exactly, since
+// min and the factors 2 and 1/2 do not round.  That form needs no products
+// of signs.
 __device__ __forceinline__ double slope(double x, double y, double z) {
-  return minmod(THETA * (y - x), minmod((z - x) * 0.5, THETA * (z - y)));
+  const double dl = THETA * (y - x), dc = (z - x) * 0.5, dr = THETA * (z - y);
+  const double m = fmin(fabs(dl), fmin(fabs(dc), fabs(dr)));
+  const bool nl = dl < 0.0, nc = dc < 0.0, nr = dr < 0.0;
+  const bool same = (nl == nc) & (nc == nr);
+  return same ? (nl ? -m : m) : 0.0;
+}
+
+// ------------------------------------------------ division, square root
+// The IEEE-exact expansions of a/b and sqrt(x) on gfx950 cost ~11 and ~15
+// VALU instructions.  These start from the hardware approximations
+// (v_rcp_f64, v_rsq_f64, about 2^-26 relative) and refine with FMA Newton
+// steps to within about one ulp.  Operands here are finite, normal and
+// nonzero in any physical state; special values are not treated.
+__device__ __forceinline__ double rcp_nr(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double div_nr(double a, double b) {
+  const double r = rcp_nr(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);  // ~ 1/sqrt(x)
+  double g = x * y, h = 0.5 * y;
+  double e = fma(-g, h, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  e = fma(-g, h, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  const double d = fma(-g, g, x);
+  return x == 0.0 ? x : fma(d, h, g);
 }
 
 // ---------------------------------------------------------------- EoS
@@ -87,10 +127,10 @@ __device__ __forceinline__ TabPos tab_locate(const EosTable &T, double e) {
     }
   } else if (e >= T.e_last) {  // beyond the table, +inf included
     r.k = T.n - 2;
-    r.fr = (e - T.e_pen) / (T.e_last - T.e_pen);
+    r.fr = (e - T.e_pen) * T.inv_last_span;
   } else {  // below the first node, or NaN
     r.k = 0;
-    r.fr = (e - T.e_first) / (T.e_second - T.e_first);
+    r.fr = (e - T.e_first) * T.inv_first_span;
   }
   return r;
 }
@@ -198,7 +238,33 @@ __device__ __forceinline__ double zeta_over_s(double T, double tpeak) {
 
 // ------------------------------------------------------------ root-find
 // energyDensityFromConservedVariables, EnergyMomentumTensor.cu:19-48:
-// Newton on e with the caller's guess, stopping at |de| <= 1e-3 |e|.
+// Newton on e with the caller's guess, stopping at |de| <= 1e-3 |e|; the
+// step is damped by 0.9 after MAX_ITERS/2 iterations.  The update
+// e0 - f/fp with fp = 1 - num/den is evaluated as e0 - f den/(den - num),
+// and 1/e0 is shared: three divisions per iteration instead of five, equal
+// to the reference's expression up to rounding.
+template <int EOS, bool DAMPED>
+__device__ __forceinline__ bool newton_step(const EosTable &T, double &e0, double M0, double M,
+                                            double Pi) {
+  double p, cs2;
+  eos_p_cs2<EOS>(T, e0, p, cs2);
+  const double ie0 = rcp_nr(e0);
+  const double cst2 = p * ie0;
+  const double A = fma(M0, 1.0 - cst2, Pi);
+  const double B = fma(M0, M0 + Pi, -M);
+  const double H = sqrt_nr(fabs(A * A + 4.0 * cst2 * B));
+  const double D = div_nr(A - H, 2.0 * cst2);
+  const double f = e0 + D;
+  const double dc = cs2 - cst2;
+  const double num = dc * (B + D * H - dc * cst2 * D * M0 * ie0);
+  const double den = cst2 * e0 * H;
+  const double step = div_nr((DAMPED ? 0.9 * f : f) * den, den - num);
+  const double e = e0 - step;
+  const bool done = fabs(e - e0) <= 0.001 * fabs(e);
+  e0 = e;
+  return done;
+}
+
 template <int EOS>
 __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, double M0,
                                           double M, double Pi,
@@ -206,24 +272,13 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
   if constexpr (EOS == EOS_CONFORMAL) {
     return fabs(sqrt(fabs(4.0 * M0 * M0 - 3.0 * M)) - M0);
   } else {
-    double e0 = ePrev;
-    for (int it = 0; it < MAX_ITERS; ++it) {
-      double p, cs2;
-      eos_p_cs2<EOS>(T, e0, p, cs2);
-      double cst2 = p / e0;
-      double A = fma(M0, 1.0 - cst2, Pi);
-      double B = fma(M0, M0 + Pi, -M);
-      double H = sqrt(fabs(A * A + 4.0 * cst2 * B));
-      double D = (A - H) / (2.0 * cst2);
-      double f = e0 + D;
-      double dc = cs2 - cst2;
-      double fp = 1.0 - (dc * (B + D * H - (dc * cst2 * D * M0) / e0)) / (cst2 * e0 * H);
-      double e = (it < MAX_ITERS / 2) ? e0 - f / fp : e0 - 0.9 * f / fp;
-      if (fabs(e - e0) <= 0.001 * fabs(e)) return e;
-      e0 = e;
-    }
+    double e = ePrev;
+    for (int it = 0; it < MAX_ITERS / 2; ++it)
+      if (newton_step<EOS, false>(T, e, M0, M, Pi)) return e;
+    for (int it = MAX_ITERS / 2; it < MAX_ITERS; ++it)
+      if (newton_step<EOS, true>(T, e, M0, M, Pi)) return e;
     atomicAdd(&diag[DIAG_NOT_CONVERGED], 1ull);
-    return e0;
+    return e;  // the last iterate, as the reference returns e0 after e0 = e
   }
 }
 
@@ -258,6 +313,33 @@ __device__ __forceinline__ Prim inferred(const EosTable &T, double t, double ttt
   r.uy = M2 * E2;
   r.un = M3 * E2;
   return r;
+}
+
+// Flux velocity v = u_dir/u^tau of a half-site state, as getInferredVariables
+// (EnergyMomentumTensor.cu:51-108) followed by u_dir/u^tau
+// (SpectralRadius.cu, FluxFunctions.cu).  With E = 1/(e+P),
+// u^tau = sqrt|(M0+P)E| and u_dir = M_dir E/u^tau, so
+// v = M_dir E/|(M0+P)E| = sign(e+P) M_dir/|M0+P|: one division and no square
+// root, equal up to rounding wherever e+P is finite and nonzero.
+template <int EOS, int DIR>
+__device__ __forceinline__ double flux_velocity(const EosTable &T, double t, double ttt,
+                                                double ttx, double tty, double ttn, double pitt,
+                                                double pitx, double pity, double pitn,
+                                                double Pi, double ePrev,
+                                                unsigned long long *diag) {
+  const double M0 = ttt - pitt, M1 = ttx - pitx, M2 = tty - pity, M3 = ttn - pitn;
+  const double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+  double e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag);
+  if (isnan(e)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
+  double p = eos_p<EOS>(T, e);
+  if (e < 1.e-7) {
+    e = 1.e-7;
+    p = 1.e-7;
+    atomicAdd(&diag[DIAG_E_CLAMPED], 1ull);
+  }
+  const double P = p + Pi;
+  const double Md = DIR == 0 ? M1 : (DIR == 1 ? M2 : M3);
+  return ((e + P > 0.0) ? Md : -Md) / fabs(M0 + P);
 }
 
 }  // namespace gvh

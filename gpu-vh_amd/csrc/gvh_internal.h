@@ -54,6 +54,8 @@ struct gvh_context {
   int iu = 0;   // buffer holding u^n
   int iup = 1;  // buffer holding u^{n-1}
   int ie = 0;   // (e, p) buffer holding the current state
+  int variant = 0;  // stage kernel arrangement (gvh_stage.hip launch_z)
+  int strip = 16;   // y rows per traversal strip
   bool initialized = false;
 
   int iQ() const { return 1 - iq; }

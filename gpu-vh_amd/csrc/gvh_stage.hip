@@ -244,118 +244,251 @@ constexpr int BLOCK = 256;
 // half-site states inverted using this cell's e as the guess, plus the
 // central-difference pi / Pi gradient sources of loadSourceTerms{X,Y,Z}
 // (SourceTerms.cu:444-567).  acc[n] += ((H_b - H_f)/h + S_dir) dt.
-template <int EOS, bool BULK, int DIR>
-__device__ __forceinline__ void direction_update(const StageArgs &A, long long s,
-                                                 double ePrev, double ut_s,
-                                                 double udir_s, double *acc) {
+//
+// Half-site states (HalfSiteExtrapolation.cu:16-29): right/left at i+1/2
+// are q_{i+1} - D_{i+1}/2 and q_i + D_i/2, right/left at i-1/2 are
+// q_i - D_i/2 and q_{i-1} + D_{i-1}/2.  The nine variables the inversion
+// needs (T^{tau mu}, pi^{tau mu}, Pi) are reconstructed state by state for
+// the four solves, then every variable once more for its fluxes.
+// Every stencil load is a uniform (scalar) base plus the thread's 32-bit
+// cell offset.
+template <int EOS, bool BULK, int DIR, bool KEEP_STATES>
+__device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s, double ePrev,
+                                                 double v_s, double *acc) {
   constexpr int NV = BULK ? 15 : 14;
   const long long st = DIR == 0 ? 1 : (DIR == 1 ? (long long)A.ncx : (long long)A.ncx * A.ncy);
-  const double *q = A.qin;
+  const double *__restrict__ q = A.qin;
   const long long G = A.G;
   const double t = A.t;
+  const double hc = A.half_inv_h[DIR];
 
-  // pass 1: the four half-site states, one after another.  state 0: right
-  // state at i+1/2 (q_{i+1} - D_{i+1}/2); 1: left at i+1/2 (q_i + D_i/2);
-  // 2: right at i-1/2 (q_i - D_i/2); 3: left at i-1/2 (q_{i-1} + D_{i-1}/2)
-  // (HalfSiteExtrapolation.cu:16-29).  Only T^{tau mu}, pi^{tau mu} and Pi
-  // enter the inversion; its flux velocity v = u_dir/u^tau is kept.
-  double vRf = 0.0, vLf = 0.0, vRb = 0.0, vLb = 0.0;
-#pragma unroll 1
-  for (int state = 0; state < 4; ++state) {
-    const long long o = state == 0 ? st : (state == 3 ? -st : 0);
-    const double sg = (state & 1) ? 0.5 : -0.5;
-    double w[9];
+  double vRf, vLf, vRb, vLb;
+  double Rf[9], Lf[9], Rb[9], Lb[9];  // used when KEEP_STATES
+  double cdv[5];
+  if constexpr (KEEP_STATES) {
+    // all four states of the nine inversion variables stay in registers
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       if (a == 8 && !BULK) {
-        w[8] = 0.0;
+        Rf[8] = Lf[8] = Rb[8] = Lb[8] = cdv[4] = 0.0;
         continue;
       }
       const int n = a < 8 ? a : 14;
-      const double *b = q + n * G + s + o;
-      const double xm = b[-st], x0 = b[0], xp = b[st];
-      w[a] = x0 + slope(xm, x0, xp) * sg;
+      const double *b = q + n * G;
+      const double vm2 = (b - 2 * st)[s], vm1 = (b - st)[s], v0 = b[s], vp1 = (b + st)[s],
+                   vp2 = (b + 2 * st)[s];
+      const double D0 = slope(vm1, v0, vp1);
+      Rf[a] = vp1 - slope(v0, vp1, vp2) * 0.5;
+      Lf[a] = v0 + D0 * 0.5;
+      Rb[a] = v0 - D0 * 0.5;
+      Lb[a] = vm1 + slope(vm2, vm1, v0) * 0.5;
+      if (a >= 4) cdv[a - 4] = (vp1 - vm1) * hc;
     }
-    const Prim pr = inferred<EOS>(A.tab, t, w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
-                                  ePrev, A.diag);
-    const double ud = DIR == 0 ? pr.ux : (DIR == 1 ? pr.uy : pr.un);
-    const double v = ud / pr.ut;  // spectral radius |v| (SpectralRadius.cu:18-30)
-    vRf = state == 0 ? v : vRf;
-    vLf = state == 1 ? v : vLf;
-    vRb = state == 2 ? v : vRb;
-    vLb = state == 3 ? v : vLb;
+    vRf = flux_velocity<EOS, DIR>(A.tab, t, Rf[0], Rf[1], Rf[2], Rf[3], Rf[4], Rf[5], Rf[6],
+                                  Rf[7], Rf[8], ePrev, A.diag);
+    vLf = flux_velocity<EOS, DIR>(A.tab, t, Lf[0], Lf[1], Lf[2], Lf[3], Lf[4], Lf[5], Lf[6],
+                                  Lf[7], Lf[8], ePrev, A.diag);
+    vRb = flux_velocity<EOS, DIR>(A.tab, t, Rb[0], Rb[1], Rb[2], Rb[3], Rb[4], Rb[5], Rb[6],
+                                  Rb[7], Rb[8], ePrev, A.diag);
+    vLb = flux_velocity<EOS, DIR>(A.tab, t, Lb[0], Lb[1], Lb[2], Lb[3], Lb[4], Lb[5], Lb[6],
+                                  Lb[7], Lb[8], ePrev, A.diag);
+  } else {
+    // the four states one after another (rolled: few live registers)
+    vRf = vLf = vRb = vLb = 0.0;
+#pragma unroll 1
+    for (int state = 0; state < 4; ++state) {
+      const long long o = state == 0 ? st : (state == 3 ? -st : 0);
+      const double sg = (state & 1) ? 0.5 : -0.5;
+      double w[9];
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        if (a == 8 && !BULK) {
+          w[8] = 0.0;
+          continue;
+        }
+        const int n = a < 8 ? a : 14;
+        const double *b = q + n * G + o;
+        const double xm = (b - st)[s], x0 = b[s], xp = (b + st)[s];
+        w[a] = x0 + slope(xm, x0, xp) * sg;
+      }
+      const double v = flux_velocity<EOS, DIR>(A.tab, t, w[0], w[1], w[2], w[3], w[4], w[5], w[6],
+                                               w[7], w[8], ePrev, A.diag);
+      vRf = state == 0 ? v : vRf;
+      vLf = state == 1 ? v : vLf;
+      vRb = state == 2 ? v : vRb;
+      vLb = state == 3 ? v : vLb;
+    }
+#pragma unroll
+    for (int a = 4; a < 9; ++a) {
+      if (a == 8 && !BULK) {
+        cdv[4] = 0.0;
+        continue;
+      }
+      const int n = a < 8 ? a : 14;
+      cdv[a - 4] = ((q + n * G + st)[s] - (q + n * G - st)[s]) * hc;
+    }
   }
-  // local propagation speeds (LocalPropagationSpeed.cu:16-24)
+  // local propagation speeds, max(|v_L|, |v_R|) (LocalPropagationSpeed.cu:16-24)
   const double aF = fmax(fabs(vLf), fabs(vRf));
   const double aB = fmax(fabs(vLb), fabs(vRb));
 
-  // directional sources from central differences of pi and Pi
-  const double hc = A.half_inv_h[DIR];
-  auto cd = [&](int n) { return (q[n * G + s + st] - q[n * G + s - st]) * hc; };
-  const double v = udir_s / ut_s;
+  // directional sources: central differences of pi^{tau mu} and Pi and of
+  // the three spatial pi components of this direction
+  auto cd = [&](int n) { return ((q + n * G + st)[s] - (q + n * G - st)[s]) * hc; };
+  const double cPi = cdv[4];
   double Sd0, Sd1, Sd2, Sd3;
   if constexpr (DIR == 0) {
-    const double dPi = BULK ? cd(14) : 0.0;
-    Sd0 = cd(4) * v - cd(5) - (BULK ? v * dPi : 0.0);
-    Sd1 = cd(5) * v - cd(8) - dPi;
-    Sd2 = cd(6) * v - cd(9);
-    Sd3 = cd(7) * v - cd(10);
+    Sd0 = cdv[0] * v_s - cdv[1] - (BULK ? v_s * cPi : 0.0);
+    Sd1 = cdv[1] * v_s - cd(8) - cPi;
+    Sd2 = cdv[2] * v_s - cd(9);
+    Sd3 = cdv[3] * v_s - cd(10);
   } else if constexpr (DIR == 1) {
-    const double dPi = BULK ? cd(14) : 0.0;
-    Sd0 = cd(4) * v - cd(6) - (BULK ? v * dPi : 0.0);
-    Sd2 = cd(6) * v - cd(11) - dPi;
-    Sd1 = cd(5) * v - cd(9);
-    Sd3 = cd(7) * v - cd(12);
+    Sd0 = cdv[0] * v_s - cdv[2] - (BULK ? v_s * cPi : 0.0);
+    Sd2 = cdv[2] * v_s - cd(11) - cPi;
+    Sd1 = cdv[1] * v_s - cd(9);
+    Sd3 = cdv[3] * v_s - cd(12);
   } else {
-    const double dPi = BULK ? cd(14) : 0.0;
-    Sd0 = cd(4) * v - cd(7) - (BULK ? v * dPi : 0.0);
-    Sd3 = cd(7) * v - cd(13) - (BULK ? dPi / (t * t) : 0.0);
-    Sd1 = cd(5) * v - cd(10);
-    Sd2 = cd(6) * v - cd(12);
+    Sd0 = cdv[0] * v_s - cdv[3] - (BULK ? v_s * cPi : 0.0);
+    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? cPi / (t * t) : 0.0);
+    Sd1 = cdv[1] * v_s - cd(10);
+    Sd2 = cdv[2] * v_s - cd(12);
   }
 
-  // pass 2: interface fluxes of every variable
-  const double ih = A.inv_h[DIR];
-#pragma unroll 1
-  for (int n = 0; n < NV; ++n) {
-    const double *b = q + n * G + s;
-    const double vm2 = b[-2 * st], vm1 = b[-st], v0 = b[0], vp1 = b[st], vp2 = b[2 * st];
-    const double Dc = slope(vm1, v0, vp1);
-    const double rf = vp1 - slope(v0, vp1, vp2) * 0.5;
-    const double lf = v0 + Dc * 0.5;
-    const double rb = v0 - Dc * 0.5;
-    const double lb = vm1 + slope(vm2, vm1, v0) * 0.5;
-    const double Hf = (vRf * rf + vLf * lf - aF * (rf - lf)) * 0.5;
-    const double Hb = (vRb * rb + vLb * lb - aB * (rb - lb)) * 0.5;
-    double r = (Hb - Hf) * ih;
-    if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
-    acc[n * BLOCK] += r * A.dt;
+  const double ih = A.inv_h[DIR], dt = A.dt;
+  // H = (F(q_R) + F(q_L) - a (q_R - q_L))/2 with F = v q
+#define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
+  (((vRb * (rb) + vLb * (lb) - aB * ((rb) - (lb))) * 0.5 -                   \
+    (vRf * (rf) + vLf * (lf) - aF * ((rf) - (lf))) * 0.5) * ih)
+  int n0 = 0;
+  if constexpr (KEEP_STATES) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      if (a == 8 && !BULK) continue;
+      const int n = a < 8 ? a : 14;
+      double r = GVH_FLUXDIFF(Rf[a], Lf[a], Rb[a], Lb[a]);
+      if (a == 0) r += Sd0;
+      if (a == 1) r += Sd1;
+      if (a == 2) r += Sd2;
+      if (a == 3) r += Sd3;
+      acc[n * BLOCK] += r * dt;
+    }
+    n0 = 8;
   }
+  const int n1 = KEEP_STATES ? 14 : NV;
+#pragma unroll 1
+  for (int n = n0; n < n1; ++n) {
+    const double *b = q + n * G;
+    const double vm2 = (b - 2 * st)[s], vm1 = (b - st)[s], v0 = b[s], vp1 = (b + st)[s],
+                 vp2 = (b + 2 * st)[s];
+    const double D0 = slope(vm1, v0, vp1);
+    const double rf = vp1 - slope(v0, vp1, vp2) * 0.5;
+    const double lf = v0 + D0 * 0.5;
+    const double rb = v0 - D0 * 0.5;
+    const double lb = vm1 + slope(vm2, vm1, v0) * 0.5;
+    double r = GVH_FLUXDIFF(rf, lf, rb, lb);
+    if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
+    acc[n * BLOCK] += r * dt;
+  }
+#undef GVH_FLUXDIFF
 }
 
-// ------------------------------------------------------ stage kernel
-template <int EOS, bool BULK, bool STAGE2, bool HAS_Z>
-__global__ __launch_bounds__(BLOCK) void kt_stage_kernel(StageArgs A) {
+// Cell of a thread.  Traversal order: x fastest, then the rows of a strip
+// of A.strip rows in y, then eta, then the strips -- so the eta neighbours
+// of a cell (+-2 planes) are a few strips-of-rows away in launch order rather
+// than a whole x-y plane, and stay in L2.  Blocks are renumbered so that each
+// of the 8 XCDs (which receive consecutive workgroups round-robin) gets one
+// contiguous run of that order.  The grid is a multiple of 8 blocks; threads
+// past the lattice (or in the padding rows of the last strip) exit.
+struct CellPos {
+  int i, j, k;
+  unsigned s;
+  bool valid;
+};
+
+__device__ __forceinline__ CellPos cell_of_thread(const StageArgs &A) {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  const int by = A.strip;
+  const long long tid = (long long)lb * BLOCK + threadIdx.x;
+  const int i = (int)(tid % A.nx);
+  const long long r = tid / A.nx;
+  const int jj = (int)(r % by);
+  const long long r2 = r / by;
+  const int k = (int)(r2 % A.nz);
+  const int yb = (int)(r2 / A.nz);
+  const int j = yb * by + jj;
+  CellPos c;
+  c.valid = j < A.ny && yb < A.nstrips;
+  c.i = i + NG;
+  c.j = (c.valid ? j : 0) + NG;
+  c.k = k + NG;
+  c.s = (unsigned)(c.i + (long long)A.ncx * (c.j + (long long)A.ncy * c.k));
+  return c;
+}
+
+// Source part of the stage (eulerStepKernelSource_1D, EulerStep.cu:947-1009):
+// acc = q + dt S written to the stage's output buffer at the cell.
+template <int EOS, bool BULK, bool HAS_Z>
+__global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
+  constexpr int NV = BULK ? 15 : 14;
+  const CellPos c = cell_of_thread(A);
+  if (!c.valid) return;
+  const unsigned s = c.s;
+  const long long sy = A.ncx, sz = (long long)A.ncx * A.ncy;
+  const long long G = A.G;
+  const double *u = A.uin;
+  const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
+  const double e_s = A.ein[s], p_s = A.pin[s];
+  // minmod gradients of u and p (SourceTerms.cu:662-719)
+  Grad g;
+  const long long st[3] = {1, sy, sz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d == 2 && !HAS_Z) {
+      g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = 0.0;
+      continue;
+    }
+    const double ih = A.inv_h[d];
+    const double *um = u - st[d], *upl = u + st[d];
+    g.ut[d] = slope(um[s], ut, upl[s]) * ih;
+    g.ux[d] = slope((um + G)[s], ux, (upl + G)[s]) * ih;
+    g.uy[d] = slope((um + 2 * G)[s], uy, (upl + 2 * G)[s]) * ih;
+    g.un[d] = slope((um + 3 * G)[s], un, (upl + 3 * G)[s]) * ih;
+    g.p[d] = slope((A.pin - st[d])[s], p_s, (A.pin + st[d])[s]) * ih;
+  }
+  double Q[15], S[15];
+#pragma unroll
+  for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? (A.qin + n * G)[s] : 0.0;
+  S[14] = 0.0;
+  const double *up = A.uprev;
+  source_terms<EOS, BULK>(A, Q, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s], (up + 2 * G)[s],
+                          (up + 3 * G)[s], g, S);
+#pragma unroll
+  for (int n = 0; n < NV; ++n) (A.qout + n * G)[s] = Q[n] + A.dt * S[n];
+}
+
+// Flux part of the stage plus the cell's finishing work: the three
+// directional updates in the reference's order, the convex average (stage
+// 2), the root-find, regulation, and the writes of the cell and its ghost
+// copies.  acc starts from what kt_source_kernel stored at the cell.
+// FUSED: the source terms are computed here first (one kernel per stage);
+// otherwise kt_source_kernel has stored acc = q + dt S at the cell.
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP>
+__global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
   constexpr int NV = BULK ? 15 : 14;
   __shared__ double acc_lds[15 * BLOCK];
   double *acc = acc_lds + threadIdx.x;
-  const int nint = A.nx * A.ny * A.nz;
-  const int tid = blockIdx.x * BLOCK + threadIdx.x;
-  if (tid >= nint) return;
-  const int i = tid % A.nx + NG;
-  const int rem = tid / A.nx;
-  const int j = rem % A.ny + NG;
-  const int k = rem / A.ny + NG;
+  const CellPos c = cell_of_thread(A);
+  if (!c.valid) return;
+  const int i = c.i, j = c.j, k = c.k;
+  const unsigned s = c.s;
   const long long sy = A.ncx, sz = (long long)A.ncx * A.ncy;
-  const long long s = i + sy * j + sz * k;
   const long long G = A.G;
-
   const double *u = A.uin;
-  const double ut = u[s], ux = u[G + s], uy = u[2 * G + s], un = u[3 * G + s];
-  const double e_s = A.ein[s], p_s = A.pin[s];
-
-  {
-    // minmod gradients of u and p (SourceTerms.cu:662-719)
+  const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
+  const double e_s = A.ein[s];
+  if constexpr (FUSED) {
+    const double p_s = A.pin[s];
     Grad g;
     const long long st[3] = {1, sy, sz};
 #pragma unroll
@@ -365,31 +498,35 @@ __global__ __launch_bounds__(BLOCK) void kt_stage_kernel(StageArgs A) {
         continue;
       }
       const double ih = A.inv_h[d];
-      const long long a = s - st[d], b = s + st[d];
-      g.ut[d] = slope(u[a], ut, u[b]) * ih;
-      g.ux[d] = slope(u[G + a], ux, u[G + b]) * ih;
-      g.uy[d] = slope(u[2 * G + a], uy, u[2 * G + b]) * ih;
-      g.un[d] = slope(u[3 * G + a], un, u[3 * G + b]) * ih;
-      g.p[d] = slope(A.pin[a], p_s, A.pin[b]) * ih;
+      const double *um = u - st[d], *upl = u + st[d];
+      g.ut[d] = slope(um[s], ut, upl[s]) * ih;
+      g.ux[d] = slope((um + G)[s], ux, (upl + G)[s]) * ih;
+      g.uy[d] = slope((um + 2 * G)[s], uy, (upl + 2 * G)[s]) * ih;
+      g.un[d] = slope((um + 3 * G)[s], un, (upl + 3 * G)[s]) * ih;
+      g.p[d] = slope((A.pin - st[d])[s], p_s, (A.pin + st[d])[s]) * ih;
     }
     double Q[15], S[15];
 #pragma unroll
-    for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? A.qin[n * G + s] : 0.0;
+    for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? (A.qin + n * G)[s] : 0.0;
     S[14] = 0.0;
     const double *up = A.uprev;
-    source_terms<EOS, BULK>(A, Q, e_s, p_s, ut, ux, uy, un, up[s], up[G + s], up[2 * G + s],
-                            up[3 * G + s], g, S);
+    source_terms<EOS, BULK>(A, Q, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s],
+                            (up + 2 * G)[s], (up + 3 * G)[s], g, S);
 #pragma unroll
     for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Q[n] + A.dt * S[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (A.qout + n * G)[s];
   }
 
-  direction_update<EOS, BULK, 0>(A, s, e_s, ut, ux, acc);
-  direction_update<EOS, BULK, 1>(A, s, e_s, ut, uy, acc);
-  if constexpr (HAS_Z) direction_update<EOS, BULK, 2>(A, s, e_s, ut, un, acc);
+  // flux velocity of the cell itself for the directional sources (v = u/u^tau)
+  direction_update<EOS, BULK, 0, KEEP>(A, s, e_s, ux / ut, acc);
+  direction_update<EOS, BULK, 1, KEEP>(A, s, e_s, uy / ut, acc);
+  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP>(A, s, e_s, un / ut, acc);
 
   if constexpr (STAGE2) {
 #pragma unroll 1
-    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + A.qn[n * G + s]) * 0.5;
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + (A.qn + n * G)[s]) * 0.5;
   }
 
   // inferred variables with this cell's e as the guess
@@ -445,7 +582,7 @@ __global__ __launch_bounds__(BLOCK) void kt_stage_kernel(StageArgs A) {
 
   auto put = [&](double *base, double val) {
     base[s] = val;
-    for (int m = 0; m < ng; ++m) base[s + gh[m]] = val;
+    for (int m = 0; m < ng; ++m) base[(long long)s + gh[m]] = val;
   };
 #pragma unroll 1
   for (int n = 0; n < NV; ++n) put(A.qout + n * G, acc[n * BLOCK]);
@@ -458,16 +595,34 @@ __global__ __launch_bounds__(BLOCK) void kt_stage_kernel(StageArgs A) {
 }
 
 // ------------------------------------------------------------ launch
+// Kernel arrangement of a stage (StageArgs::variant): 0 fused kernel with
+// the inversion states kept in registers, 1 fused with states rebuilt one at
+// a time, 2 separate source kernel + flux kernel (states rebuilt).
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z>
+static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
+  const long long ncell = (long long)A.nx * A.strip * A.nstrips * A.nz;
+  const int block = BLOCK;
+  const int grid = (int)(((ncell + block - 1) / block + 7) / 8 * 8);  // multiple of 8 (XCD order)
+  switch (A.variant) {
+    case 0:
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true>), dim3(grid),
+                         dim3(block), 0, st, A);
+      break;
+    case 1:
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, false>), dim3(grid),
+                         dim3(block), 0, st, A);
+      break;
+    default:
+      hipLaunchKernelGGL((kt_source_kernel<EOS, BULK, HAS_Z>), dim3(grid), dim3(block), 0, st, A);
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, false, false>), dim3(grid),
+                         dim3(block), 0, st, A);
+  }
+  return hipGetLastError();
+}
+
 template <int EOS, bool BULK, bool STAGE2>
 static hipError_t launch_t(const StageArgs &A, bool has_z, hipStream_t st) {
-  const int nint = A.nx * A.ny * A.nz;
-  const int block = BLOCK;
-  const int grid = (nint + block - 1) / block;
-  if (has_z)
-    hipLaunchKernelGGL((kt_stage_kernel<EOS, BULK, STAGE2, true>), dim3(grid), dim3(block), 0, st, A);
-  else
-    hipLaunchKernelGGL((kt_stage_kernel<EOS, BULK, STAGE2, false>), dim3(grid), dim3(block), 0, st, A);
-  return hipGetLastError();
+  return has_z ? launch_z<EOS, BULK, STAGE2, true>(A, st) : launch_z<EOS, BULK, STAGE2, false>(A, st);
 }
 
 template <int EOS>

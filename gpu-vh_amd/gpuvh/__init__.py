@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libgpuvh.so")
+# GPUVH_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GPUVH_LIB") or os.path.join(PKG_DIR, "lib", "libgpuvh.so")
 DATA_DIR = os.path.join(PKG_DIR, "data")
 EOS_TABLE_PATH = os.path.join(DATA_DIR, "eos_qcdlike_2000.dat")
 

@@ -12,7 +12,15 @@ PI_FIELDS = [gpuvh.PITT, gpuvh.PITX, gpuvh.PITY, gpuvh.PITN, gpuvh.PIXX, gpuvh.P
 U_FIELDS = [gpuvh.UT, gpuvh.UX, gpuvh.UY, gpuvh.UN]
 
 
-def rel_err(x, ref, floor_frac=1e-6, scale=None):
+# Floor of the per-cell relative error, as a fraction of the field's tensor
+# scale (floor_scales).  A component much smaller than its tensor is a sum of
+# O(scale) terms that cancel, so its rounding noise is ~1e-15 x scale whatever
+# the operation order; below 1e-4 x scale the 1e-10 relative bar therefore
+# becomes an absolute bar of 1e-14 x scale.
+FLOOR_FRAC = 1e-4
+
+
+def rel_err(x, ref, floor_frac=FLOOR_FRAC, scale=None):
     """max_cells |x - ref| / max(|ref|, floor_frac * scale).
 
     scale defaults to the field's own maximum; a field that is zero
@@ -32,9 +40,9 @@ ETA_INDICES = {gpuvh.UN: 1, gpuvh.PITN: 1, gpuvh.PIXN: 1, gpuvh.PIYN: 1, gpuvh.P
 
 def floor_scales(ref, tau):
     """Per-field scale for the relative-error floor: components of u^mu and
-    of pi^{mu nu} are floored at 1e-6 of the largest component of the same
-    tensor in the orthonormal frame (tau^n_eta x component), converted back
-    to the field's own units.  Components that vanish by symmetry (u^eta,
+    of pi^{mu nu} are floored at FLOOR_FRAC of the largest component of the
+    same tensor in the orthonormal frame (tau^n_eta x component), converted
+    back to the field's own units.  Components that vanish by symmetry (u^eta,
     pi^{x eta}, ... on the eta = 0 plane) are computed as sums of O(1) terms
     that cancel, so their rounding noise is set by the tensor's scale, not by
     their own.  Scalars (e, Pi) use their own maximum."""
@@ -133,7 +141,7 @@ def compare(orc, ctx, fields, tau, report=None):
         name = gpuvh.FIELD_NAMES[f]
         out[name] = rel_err(got[f], ref[f], scale=scales[f])
         if report is not None and scales[f] > 0:
-            den = np.maximum(np.abs(ref[f]), 1e-6 * scales[f])
+            den = np.maximum(np.abs(ref[f]), FLOOR_FRAC * scales[f])
             r = np.abs(got[f] - ref[f]) / den
             w = np.unravel_index(np.argmax(r), r.shape)
             report.append((name, out[name], w, float(ref[f][w]), float(got[f][w] - ref[f][w]),
