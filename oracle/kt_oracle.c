@@ -589,8 +589,38 @@ static inline long idx(const kto_ctx *c, int i, int j, int k) {
   return i + (long)c->ncx * (j + (long)c->ncy * k); /* DynamicalVariables.cu:33-35 */
 }
 
-/* eulerStepKernelSource_1D + loadSourceTerms2, EulerStep.cu:947-1009 and
- * SourceTerms.cu:570-755 (USE_APPROX_DERIVATIVE branch) */
+/* loadSourceTerms2, SourceTerms.cu:570-755, for one cell given its stencil
+ * derivatives: du[d][m] = d_d u^m and dp[d] = d_d p (d = x, y, eta; the
+ * USE_APPROX_DERIVATIVE minmod derivatives divided by the spacing).
+ * S[0..3]: T^{tau mu} geometric and pressure sources; S[4..14]: pi, Pi RHS. */
+void kto_source_cell(const kto_params *P, double t, const double *Q, double e,
+                     double ps, const double *u, const double *up,
+                     const double du[3][4], const double *dp, double *S) {
+  const int nv = P->bulk ? 15 : 14;
+  double Pi = P->bulk ? Q[14] : 0.0;
+  double ut = u[0], ux = u[1], uy = u[2], un = u[3];
+  /* T^{tau mu} sources, SourceTerms.cu:726-737 */
+  double pinn = Q[13];
+  double PP = ps + Pi;
+  double tnn = (e + PP) * un * un + PP / t / t + pinn; /* Tnn, EnergyMomentumTensor.cu:205-207 */
+  double vx = ux / ut, vy = uy / ut, vn = un / ut;
+  double dxvx = (du[0][1] - vx * du[0][0]) / ut;
+  double dyvy = (du[1][2] - vy * du[1][0]) / ut;
+  double dnvn = (du[2][3] - vn * du[2][0]) / ut;
+  double dkvk = dxvx + dyvy + dnvn;
+  S[0] = -(Q[0] / t + t * tnn) + dkvk * (Q[4] - ps - Pi) - vx * dp[0] - vy * dp[1] - vn * dp[2];
+  S[1] = -Q[1] / t - dp[0] + dkvk * Q[5];
+  S[2] = -Q[2] / t - dp[1] + dkvk * Q[6];
+  S[3] = -3 * Q[3] / t - dp[2] / (t * t) + dkvk * Q[7];
+  double rhs[11];
+  pimunu_sources(P, rhs, t, e, ps, ut, ux, uy, un, up[0], up[1], up[2], up[3], &Q[4], Pi, du,
+                 dkvk);
+  for (int n = 4; n < nv; ++n) S[n] = rhs[n - 4];
+  if (nv == 14) S[14] = 0.0;
+}
+
+/* eulerStepKernelSource_1D, EulerStep.cu:947-1009: q' = q + dt S with the
+ * u and p derivatives of SourceTerms.cu:662-719 (USE_APPROX_DERIVATIVE). */
 static void euler_source(kto_ctx *c, double t, double **cur, double **upd,
                          double **u, double **up) {
   const kto_params *P = &c->P;
@@ -604,33 +634,17 @@ static void euler_source(kto_ctx *c, double t, double **cur, double **upd,
         long s = idx(c, i, j, k);
         double Q[NVMAX], S[NVMAX];
         for (int n = 0; n < NVMAX; ++n) Q[n] = (n < nv) ? cur[n][s] : 0.0;
-        double Pi = P->bulk ? Q[14] : 0.0;
-        double ps = p[s], ut = u[0][s], ux = u[1][s], uy = u[2][s], un = u[3][s];
         const long st[3] = {sx, sy, sz};
         const double h[3] = {P->dx, P->dy, P->dz};
         double du[3][4], dp[3];
         for (int d = 0; d < 3; ++d) {
           for (int m = 0; m < 4; ++m)
             du[d][m] = kto_approx_deriv(u[m][s - st[d]], u[m][s], u[m][s + st[d]]) / h[d];
-          dp[d] = kto_approx_deriv(p[s - st[d]], ps, p[s + st[d]]) / h[d];
+          dp[d] = kto_approx_deriv(p[s - st[d]], p[s], p[s + st[d]]) / h[d];
         }
-        /* T^{tau mu} sources, SourceTerms.cu:726-737 */
-        double pinn = Q[13];
-        double PP = ps + Pi;
-        double tnn = (e[s] + PP) * un * un + PP / t / t + pinn; /* Tnn, EnergyMomentumTensor.cu:205-207 */
-        double vx = ux / ut, vy = uy / ut, vn = un / ut;
-        double dxvx = (du[0][1] - vx * du[0][0]) / ut;
-        double dyvy = (du[1][2] - vy * du[1][0]) / ut;
-        double dnvn = (du[2][3] - vn * du[2][0]) / ut;
-        double dkvk = dxvx + dyvy + dnvn;
-        S[0] = -(Q[0] / t + t * tnn) + dkvk * (Q[4] - ps - Pi) - vx * dp[0] - vy * dp[1] - vn * dp[2];
-        S[1] = -Q[1] / t - dp[0] + dkvk * Q[5];
-        S[2] = -Q[2] / t - dp[1] + dkvk * Q[6];
-        S[3] = -3 * Q[3] / t - dp[2] / (t * t) + dkvk * Q[7];
-        double rhs[11];
-        pimunu_sources(P, rhs, t, e[s], ps, ut, ux, uy, un, up[0][s], up[1][s],
-                       up[2][s], up[3][s], &Q[4], Pi, du, dkvk);
-        for (int n = 4; n < nv; ++n) S[n] = rhs[n - 4];
+        const double uc[4] = {u[0][s], u[1][s], u[2][s], u[3][s]};
+        const double upc[4] = {up[0][s], up[1][s], up[2][s], up[3][s]};
+        kto_source_cell(P, t, Q, e[s], p[s], uc, upc, (const double(*)[4])du, dp, S);
         for (int n = 0; n < nv; ++n) upd[n][s] = Q[n] + P->dt * S[n];
       }
 }
@@ -746,6 +760,36 @@ static void set_inferred(kto_ctx *c, double **q, double **u, double t) {
       }
 }
 
+/* Shear regulation factor of one cell, RegulateDissipativeCurrents.cu:69-112
+ * (xi0 = 0.1, rho_max = 1; the trace and u.pi ratios are signed, :88-92):
+ * tanh(rho)/rho when e <= e_reg, else 1.  pi[10] = tt tx ty tn xx xy xn yy yn nn. */
+double kto_regulate_factor(double t, const double *pi, double e_s, double p_s,
+                           const double *u, double e_reg) {
+  double pitt = pi[0], pitx = pi[1], pity = pi[2], pitn = pi[3], pixx = pi[4],
+         pixy = pi[5], pixn = pi[6], piyy = pi[7], piyn = pi[8], pinn = pi[9];
+  double ut = u[0], ux = u[1], uy = u[2], un = u[3];
+  double xi0 = 0.1, rhomax = 1.0;
+  double t2 = t * t;
+  double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx +
+                2 * pixy * pixy + piyy * piyy - 2 * pitn * pitn * t2 +
+                2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 + pinn * pinn * t2 * t2;
+  double spipi = sqrt(fabs(pipi));
+  double pimumu = pitt - pixx - piyy - pinn * t * t;
+  double piu0 = -pitn * t2 * un + pitt * ut - pitx * ux - pity * uy;
+  double piu1 = -pixn * t2 * un + pitx * ut - pixx * ux - pixy * uy;
+  double piu2 = -piyn * t2 * un + pity * ut - pixy * ux - piyy * uy;
+  double piu3 = -pinn * t2 * un + pitn * ut - pixn * ux - piyn * uy;
+  double a1 = (spipi / rhomax) * (1 / sqrt(e_s * e_s + 3 * p_s * p_s));
+  double den = xi0 * rhomax * spipi;
+  double a2 = pimumu / den, a3 = piu0 / den, a4 = piu1 / den, a5 = piu2 / den,
+         a6 = piu3 / den;
+  double rho = fmax(a1, fmax(a2, fmax(a3, fmax(a4, fmax(a5, a6)))));
+  double facShear = tanh(rho) / rho;
+  if (fabs(rho) < 1.e-7) facShear = 1.0;
+  if (e_s > e_reg) facShear = 1.0;
+  return facShear;
+}
+
 /* regulateDissipativeCurrents, RegulateDissipativeCurrents.cu:16-134.  The
  * Pi read is guarded by '#ifdef Pi' (never defined), so the bulk factor is
  * tanh(0)/0 -> replaced by 1 unless p is 0 or NaN, where it stays NaN. */
@@ -757,37 +801,15 @@ static void regulate(kto_ctx *c, double t, double **q, double **u) {
     for (int j = NG; j < P->ny + NG; ++j)
       for (int i = NG; i < P->nx + NG; ++i) {
         long s = idx(c, i, j, k);
-        double pitt = q[4][s], pitx = q[5][s], pity = q[6][s], pitn = q[7][s],
-               pixx = q[8][s], pixy = q[9][s], pixn = q[10][s], piyy = q[11][s],
-               piyn = q[12][s], pinn = q[13][s];
-        double ut = u[0][s], ux = u[1][s], uy = u[2][s], un = u[3][s];
+        double pi[10], uc[4];
+        for (int m = 0; m < 10; ++m) pi[m] = q[4 + m][s];
+        for (int m = 0; m < 4; ++m) uc[m] = u[m][s];
         double e_s = c->e[s], p_s = c->p[s];
-        double xi0 = 0.1, rhomax = 1.0;
-        double t2 = t * t;
-        double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx +
-                      2 * pixy * pixy + piyy * piyy - 2 * pitn * pitn * t2 +
-                      2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 +
-                      pinn * pinn * t2 * t2;
-        double spipi = sqrt(fabs(pipi));
-        double pimumu = pitt - pixx - piyy - pinn * t * t;
-        double piu0 = -pitn * t2 * un + pitt * ut - pitx * ux - pity * uy;
-        double piu1 = -pixn * t2 * un + pitx * ut - pixx * ux - pixy * uy;
-        double piu2 = -piyn * t2 * un + pity * ut - pixy * ux - piyy * uy;
-        double piu3 = -pinn * t2 * un + pitn * ut - pixn * ux - piyn * uy;
-        double a1 = (spipi / rhomax) * (1 / sqrt(e_s * e_s + 3 * p_s * p_s));
-        double den = xi0 * rhomax * spipi;
-        double a2 = pimumu / den, a3 = piu0 / den, a4 = piu1 / den,
-               a5 = piu2 / den, a6 = piu3 / den;
-        double rho = fmax(a1, fmax(a2, fmax(a3, fmax(a4, fmax(a5, a6)))));
-        double facShear = tanh(rho) / rho;
-        if (fabs(rho) < 1.e-7) facShear = 1.0;
+        double facShear = kto_regulate_factor(t, pi, e_s, p_s, uc, e_reg);
         double rhoBulk = 0.0 / p_s;
         double facBulk = tanh(rhoBulk) / rhoBulk;
         if (fabs(rhoBulk) < 1.e-7) facBulk = 1.0;
-        if (e_s > e_reg) {
-          facShear = 1.0;
-          facBulk = 1.0;
-        }
+        if (e_s > e_reg) facBulk = 1.0;
         for (int n = 4; n < 14; ++n) q[n][s] *= facShear;
         if (P->bulk) q[14][s] *= facBulk;
         c->reg[s] = facShear;

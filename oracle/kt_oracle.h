@@ -87,6 +87,15 @@ void kto_inferred(const kto_params *P, double t, const double *q, double ePrev,
  * forward (i+1/2) when fwd != 0 else backward (i-1/2); H[15]. */
 void kto_flux(const kto_params *P, int dir, const double *data, double t,
               double ePrev, int fwd, double *H);
+/* loadSourceTerms2 (T^{tau mu} sources and pi, Pi relaxation RHS) of one
+ * cell: Q[15] conserved, e, p, u[4], u_prev[4], du[3][4] = d_i u^m,
+ * dp[3] = d_i p; S[15] out. */
+void kto_source_cell(const kto_params *P, double t, const double *Q, double e,
+                     double p, const double *u, const double *up,
+                     const double du[3][4], const double *dp, double *S);
+/* shear regulation factor of one cell (1 when e > e_reg) */
+double kto_regulate_factor(double t, const double *pi, double e, double p,
+                           const double *u, double e_reg);
 /* Newton iteration count of the last kto_inferred call (non-conformal). */
 int kto_last_newton_iters(void);
 

@@ -72,6 +72,10 @@ def lib():
         L.kto_flux.argtypes = [C.POINTER(Params), C.c_int, dp, C.c_double, C.c_double,
                                C.c_int, dp]
         L.kto_last_newton_iters.restype = C.c_int
+        L.kto_regulate_factor.restype = C.c_double
+        L.kto_regulate_factor.argtypes = [C.c_double, dp, C.c_double, C.c_double, dp, C.c_double]
+        L.kto_source_cell.argtypes = [C.POINTER(Params), C.c_double, dp, C.c_double,
+                                      C.c_double, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -179,6 +183,21 @@ def inferred(params, t, q, e_prev):
     out = np.empty(6, dtype=np.float64)
     lib().kto_inferred(C.byref(params), t, _ptr(q), e_prev, _ptr(out))
     return out
+
+
+def source_cell(params, t, Q, e, p, u, up, du, dp):
+    """loadSourceTerms2 of one cell; du is (3, 4) = d_i u^m, dp (3,)."""
+    arr = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (Q, u, up, du, dp)]
+    S = np.zeros(15, dtype=np.float64)
+    lib().kto_source_cell(C.byref(params), t, _ptr(arr[0]), e, p, _ptr(arr[1]), _ptr(arr[2]),
+                          _ptr(arr[3]), _ptr(arr[4]), _ptr(S))
+    return S
+
+
+def regulate_factor(t, pi10, e, p, u, e_reg):
+    pi = np.ascontiguousarray(np.asarray(pi10, dtype=np.float64))
+    uu = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    return lib().kto_regulate_factor(t, _ptr(pi), e, p, _ptr(uu), e_reg)
 
 
 def flux(params, direction, data, t, e_prev, fwd):
