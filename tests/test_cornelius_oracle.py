@@ -1,0 +1,110 @@
+"""Pins of the cornelius restatement (oracle/cornelius_oracle.cpp) against
+the reference's golden outputs and, where it builds, the reference's own
+cornelius.  CPU only."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import cornelius_lib as CL
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornelius")
+
+
+def fmt(x):
+    """std::ostream default formatting (6 significant digits)."""
+    s = "%g" % x
+    return s
+
+
+def example_inputs():
+    """Corner values of example_cornelius.cpp: 10000 squares (2x2), then 10000
+    cubes (2x2x2), then 10000 hypercubes (2x2x2x2), all from one rand()
+    stream, row-major."""
+    seq = CL.glibc_rand_sequence(10000 * (4 + 8 + 16))
+    sq = seq[:40000].reshape(10000, 4)
+    cu = seq[40000:120000].reshape(10000, 8)
+    hc = seq[120000:].reshape(10000, 16)
+    return sq, cu, hc
+
+
+@pytest.fixture(scope="module")
+def inputs():
+    return example_inputs()
+
+
+def test_glibc_sequence_known_start():
+    s = CL.glibc_rand_sequence(3) * 2147483647.0
+    assert list(np.round(s).astype(int)) == [1804289383, 846930886, 1681692777]
+
+
+@pytest.mark.parametrize("dim", [3, 4])
+def test_golden_outputs(inputs, dim):
+    """Every element of the 10000 random cells, centroid then normal, printed
+    like the reference, equals output_{3d,4d}.dat line for line."""
+    cells = inputs[dim - 2]
+    lines = []
+    for c in cells:
+        cen, nrm = CL.find(dim, c, np.ones(dim), 0.5)
+        for i in range(len(cen)):
+            lines.append(" ".join(fmt(x) for x in list(cen[i]) + list(nrm[i])) + " ")
+    with gzip.open(os.path.join(GOLD, "output_%dd.dat.gz" % dim), "rt") as f:
+        gold = f.read().splitlines()
+    assert len(lines) == len(gold)
+    bad = [(i, a, b) for i, (a, b) in enumerate(zip(lines, gold)) if a != b]
+    # printing at 6 digits can differ in the last digit where a value sits on
+    # a rounding boundary; those must agree numerically to 1e-5
+    for i, a, b in bad:
+        va = np.array([float(x) for x in a.split()])
+        vb = np.array([float(x) for x in b.split()])
+        assert np.allclose(va, vb, rtol=2e-5, atol=2e-6), (i, a, b)
+    assert len(bad) < 0.001 * len(gold), len(bad)
+
+
+@pytest.mark.skipif(not CL.ref_available(), reason="reference cornelius not built (oracle/_ref)")
+@pytest.mark.parametrize("dim", [3, 4])
+def test_against_reference_build(inputs, dim):
+    """Full precision against the reference's cornelius: the random cells,
+    cells with corners exactly at the iso-value, and anisotropic spacings."""
+    rng = np.random.default_rng(10)
+    cells = list(inputs[dim - 2][:3000])
+    for _ in range(2000):
+        c = rng.choice([0.2, 0.5, 0.8, 0.5, 0.5, 0.35], size=2 ** dim)
+        cells.append(c)
+    dxs = [np.ones(dim), np.array([0.01, 0.1, 0.1, 0.1][:dim])]
+    for dx in dxs:
+        for c in cells:
+            a = CL.find(dim, c, dx, 0.5)
+            b = CL.ref_find(dim, c, dx, 0.5)
+            assert len(a[0]) == len(b[0])
+            if len(b[0]) == 0:
+                continue
+            assert np.allclose(a[0], b[0], rtol=1e-12, atol=1e-14), (c, a, b)
+            assert np.allclose(a[1], b[1], rtol=1e-12, atol=1e-14 * np.abs(b[1]).max()), (c, a, b)
+
+
+def test_cell_predicate():
+    """A cell has elements iff 0 < #corners(>= v0) < 2^d."""
+    rng = np.random.default_rng(11)
+    for dim in (3, 4):
+        for _ in range(300):
+            c = rng.uniform(0, 1, 2 ** dim)
+            above = int((c >= 0.5).sum())
+            cen, _ = CL.find(dim, c, np.ones(dim), 0.5)
+            assert (len(cen) > 0) == (0 < above < 2 ** dim)
+
+
+def test_planar_surface_normal():
+    """A linear field crosses the cell in a plane: one element whose normal
+    points down the gradient with |normal| = the plane's area (volume) in
+    units of the spacing."""
+    dx = np.array([0.5, 0.2, 0.3])
+    g = np.array([-1.0, 0.5, 0.2])
+    corners = np.array([0.5 + g @ (np.array([a, b, e]) * dx - dx / 2)
+                        for a in (0, 1) for b in (0, 1) for e in (0, 1)])
+    cen, nrm = CL.find(3, corners, dx, 0.5)
+    assert len(cen) == 1
+    n = nrm[0] / np.linalg.norm(nrm[0])
+    assert np.allclose(n, -g / np.linalg.norm(g), atol=1e-12)
+    assert np.allclose(cen[0], dx / 2, atol=1e-12)
