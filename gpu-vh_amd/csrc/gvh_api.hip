@@ -43,13 +43,6 @@ int gvh_fail(int code, const char *fmt, ...) {
 
 extern "C" const char *gvh_last_error(void) { return g_err.c_str(); }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t _e = (expr);                                                        \
-    if (_e != hipSuccess)                                                          \
-      return gvh_fail(GVH_EHIP, "%s failed: %s", #expr, hipGetErrorString(_e));  \
-  } while (0)
-
 namespace gvh {
 // ------------------------------------------------------------- host EoS
 // Host evaluation of equilibriumEnergyDensity (EquationOfState.cu:151-204)
@@ -317,6 +310,7 @@ extern "C" int gvh_destroy(gvh_context *c) {
   if (c->count) (void)hipFree(c->count);
   if (c->tab_pc) (void)hipFree(c->tab_pc);
   if (c->tab_tt) (void)hipFree(c->tab_tt);
+  fo_free(c->fo);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GVH_OK;
@@ -395,6 +389,7 @@ extern "C" int gvh_set_initial_state(gvh_context *c, double t0, const double *e,
   HIP_TRY(hipMemcpyAsync(c->ubuf[c->iup], hup.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->initialized = true;
+  c->steps_done = 0;
   return GVH_OK;
 }
 
@@ -491,6 +486,7 @@ static void rotate_after_step(gvh_context *c) {
   c->iu = c->iup;   // u^{n+1} was written into the u^{n-1} buffer
   c->iup = old_u;   // u^n becomes the previous-step velocity
   c->iq = c->iQ();  // q^{n+1} is in Q; the old q^n buffer becomes Q
+  ++c->steps_done;
 }
 
 static int step_impl(gvh_context *c, double t, float *ms1, float *ms2) {

@@ -20,10 +20,22 @@ struct HostEos {
 double host_e_of_T(const HostEos &H, double T);
 int eos_from_options(const gvh_options *o, HostEos &H);
 int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T, hipStream_t st);
+double host_eos_p(const HostEos &H, double e);
+
+// Freeze-out state of a context (gvh_freezeout.hip).
+struct FoState;
+void fo_free(FoState *f);
 
 }  // namespace gvh
 
 int gvh_fail(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return gvh_fail(GVH_EHIP, "%s failed: %s", #expr, hipGetErrorString(_e));  \
+  } while (0)
 
 struct gvh_context {
   gvh_lattice_parameters L{};
@@ -57,6 +69,8 @@ struct gvh_context {
   int variant = 0;  // stage kernel arrangement (gvh_stage.hip launch_z)
   int strip = 16;   // y rows per traversal strip
   bool initialized = false;
+  long long steps_done = 0;     // KT steps since the initial state was set
+  gvh::FoState *fo = nullptr;   // freeze-out history and surface buffers
 
   int iQ() const { return 1 - iq; }
   int iqs() const { return 2; }

@@ -26,6 +26,11 @@ FP64, MIXED = 0, 1
 # field ids, include/gpuvh.h
 (TTT, TTX, TTY, TTN, PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, PIYN, PINN, PI,
  E, P, UT, UX, UY, UN, UTP, UXP, UYP, UNP) = range(25)
+# freeze-out: surface row columns and flags (include/gpuvh.h)
+FO_COLUMNS = 20
+FO_EXACT_TAU, FO_NORMAL_TAU, FO_CURRENT_FIELDS = 1, 2, 4
+FO_VARS = ["ux", "uy", "un", "e", "pixx", "pixy", "pixn", "piyy", "piyn", "Pi"]
+
 FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy",
                "pixn", "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
                "utp", "uxp", "uyp", "unp"]
@@ -79,6 +84,18 @@ class Options(C.Structure):
     ]
 
 
+class RunOptions(C.Structure):
+    """gvh_run_options: FREQ, FOFREQ (rhic/src/HydroPlugin.cpp:39-40), freeze-out flags."""
+    _fields_ = [("freq", C.c_int), ("fofreq", C.c_int), ("fo_flags", C.c_int),
+                ("snapshots", C.c_int), ("surface", C.c_int), ("verbose", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("frozen", C.c_int), ("t_final", C.c_double),
+                ("surface_rows", C.c_longlong), ("ms_steps", C.c_double),
+                ("ms_freezeout", C.c_double), ("ms_total", C.c_double)]
+
+
 class Diagnostics(C.Structure):
     _fields_ = [("newton_not_converged", C.c_ulonglong), ("e_nan", C.c_ulonglong),
                 ("e_clamped", C.c_ulonglong)]
@@ -117,7 +134,21 @@ def lib():
         "gvh_energy_density_of_T": [C.c_void_p, C.c_double, dp],
         "gvh_equilibrium_pressure": [C.c_void_p, C.c_longlong, dp, dp],
         "gvh_dims": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "gvh_freezeout_init": [C.c_void_p, C.c_int, C.c_double, C.c_int],
+        "gvh_freezeout_record": [C.c_void_p, C.c_int],
+        "gvh_freezeout_find": [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_longlong)],
+        "gvh_freezeout_rows": [C.c_void_p, dp, C.c_longlong],
+        "gvh_freezeout_slot": [C.c_void_p, C.c_int, C.c_int, dp],
+        "gvh_cornelius_cells": [C.c_int, C.c_longlong, dp, dp, C.c_double, C.POINTER(C.c_int),
+                                dp, dp],
+        "gvh_run": [C.POINTER(LatticeParameters), C.POINTER(InitialConditionParameters),
+                    C.POINTER(HydroParameters), C.c_char_p, C.c_char_p, C.POINTER(Options),
+                    C.POINTER(RunOptions), C.POINTER(RunStats)],
+        "gvh_run_hydro": [C.c_int, C.POINTER(C.c_char_p), C.POINTER(Options),
+                          C.POINTER(RunOptions)],
     }
+    L.gvh_run_options_default.argtypes = [C.POINTER(RunOptions)]
+    L.gvh_run_options_default.restype = None
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -310,3 +341,57 @@ class Context:
         e = C.c_double()
         _check(lib().gvh_energy_density_of_T(self.h, T, C.byref(e)))
         return e.value
+
+    # -------------------------------------------------------- freeze-out
+    def freezeout_init(self, fofreq, e_fo, flags=0):
+        _check(lib().gvh_freezeout_init(self.h, fofreq, e_fo, flags))
+
+    def freezeout_record(self, n):
+        _check(lib().gvh_freezeout_record(self.h, n))
+
+    def freezeout_find(self, n, t):
+        """Search the window of loop index n; returns the (nrows, 20) rows."""
+        cnt = C.c_longlong()
+        _check(lib().gvh_freezeout_find(self.h, n, t, C.byref(cnt)))
+        rows = np.empty((cnt.value, FO_COLUMNS), dtype=np.float64)
+        if cnt.value:
+            _check(lib().gvh_freezeout_rows(self.h, _ptr(rows), cnt.value))
+        return rows
+
+    def freezeout_slot(self, logical, var):
+        out = np.empty(self.nx * self.ny * self.nz, dtype=np.float64)
+        _check(lib().gvh_freezeout_slot(self.h, logical, var, _ptr(out)))
+        return out.reshape((self.nx, self.ny, self.nz), order="F")
+
+
+def cornelius_cells(dim, corners, dx, v0):
+    """Surface elements of independent cells on the device.  corners:
+    (ncells, 2^dim), last axis fastest.  Returns (counts, centroids, normals)
+    with centroids/normals of shape (ncells, 10, dim)."""
+    v = np.ascontiguousarray(np.asarray(corners, dtype=np.float64).reshape(-1, 1 << dim))
+    n = v.shape[0]
+    d = np.ascontiguousarray(np.asarray(dx, dtype=np.float64))
+    counts = np.zeros(n, dtype=np.int32)
+    cen = np.zeros((n, 10, dim))
+    nrm = np.zeros((n, 10, dim))
+    _check(lib().gvh_cornelius_cells(dim, n, _ptr(v), _ptr(d), v0,
+                                     counts.ctypes.data_as(C.POINTER(C.c_int)), _ptr(cen),
+                                     _ptr(nrm)))
+    return counts, cen, nrm
+
+
+def run_options(freq=10, fofreq=10, fo_flags=0, snapshots=True, surface=True, verbose=False):
+    ro = RunOptions()
+    lib().gvh_run_options_default(C.byref(ro))
+    ro.freq, ro.fofreq, ro.fo_flags = freq, fofreq, fo_flags
+    ro.snapshots, ro.surface, ro.verbose = int(snapshots), int(surface), int(verbose)
+    return ro
+
+
+def run(lattice_params, ic_params, hydro_params, root_dir, output_dir, opts, ro=None):
+    """One event through the run loop (rhic/src/HydroPlugin.cpp:107-434)."""
+    st = RunStats()
+    _check(lib().gvh_run(C.byref(lattice_params), C.byref(ic_params), C.byref(hydro_params),
+                         root_dir.encode(), output_dir.encode(), C.byref(opts),
+                         C.byref(ro) if ro is not None else None, C.byref(st)))
+    return {f: getattr(st, f) for f, _ in RunStats._fields_}

@@ -181,6 +181,90 @@ int gvh_equilibrium_pressure(gvh_context *ctx, long long n, const double *e, dou
 /* Lattice geometry of the context. */
 int gvh_dims(gvh_context *ctx, int *nx, int *ny, int *nz);
 
+/* ------------------------------------------------------------ freeze-out
+ * The run loop's freeze-out finder (HydroPlugin.cpp:152-393 with
+ * freezeout/FreezeOut.cpp and cornelius), searched on the device: a ring of
+ * FOFREQ+1 time slices of (u^x, u^y, u^eta, e, pi^xx, pi^xy, pi^xeta,
+ * pi^yy, pi^yeta, Pi) in HBM, and surface rows of GVH_FO_COLUMNS doubles:
+ * tau x y eta, t*dsigma_{tau,x,y,eta}, u^x u^y u^eta, e T P,
+ * pi^xx pi^xy pi^xeta pi^yy pi^yeta, Pi (HydroPlugin.cpp:317-386). */
+#define GVH_FO_COLUMNS 20
+#define GVH_FO_MAX_FREQ 64
+
+/* Flags of gvh_freezeout_init.  0 reproduces the reference's rows. */
+#define GVH_FO_EXACT_TAU 1      /* cell tau of every window = its slice's time;
+                                   reference: one step late after the first
+                                   window (HydroPlugin.cpp:304-305)          */
+#define GVH_FO_NORMAL_TAU 2     /* normal scaled by the element's tau;
+                                   reference: by the current t (:328-331)    */
+#define GVH_FO_CURRENT_FIELDS 4 /* slices hold the current u and pi;
+                                   reference: the previous step's after an odd
+                                   number of steps, as its host copy reads the
+                                   first-allocated buffers
+                                   (DynamicalVariables.cu:310-331, 400-408,
+                                   FullyDiscreteKurganovTadmorScheme.cu:132) */
+
+/* Allocate the history ring of a context for windows of fofreq steps
+ * (FOFREQ, HydroPlugin.cpp:40) at iso-energy density e_fo.  The lattice must
+ * be 2+1D (nz == 1) or 3+1D (HydroPlugin.cpp:157-182). */
+int gvh_freezeout_init(gvh_context *ctx, int fofreq, double e_fo, int flags);
+/* Record the current state for run-loop index n >= 1: slice n % fofreq + 1,
+ * or, when n % fofreq == 0, slice 1 after slice fofreq moved to slice 0
+ * (swapAndSetHydroVariables / setHydroVariables, FreezeOut.cpp:49-126). */
+int gvh_freezeout_record(gvh_context *ctx, int n);
+/* Search the window of run-loop index n (the run loop calls it when
+ * n % fofreq == fofreq - 1, HydroPlugin.cpp:276) at current time t; returns
+ * the number of surface rows.  Slices from 2 on for n <= fofreq, else from 0. */
+int gvh_freezeout_find(gvh_context *ctx, int n, double t, long long *nelements);
+/* Copy the first nrows rows of the last search (nrows * GVH_FO_COLUMNS). */
+int gvh_freezeout_rows(gvh_context *ctx, double *rows, long long nrows);
+/* Copy variable var (0..9, order above) of logical slice `logical`. */
+int gvh_freezeout_slot(gvh_context *ctx, int logical, int var, double *host_out);
+/* Surface elements of independent cells (Cornelius::find_surface_3d/4d,
+ * cornelius.cpp:2161-2287) on the device: corners[ncells][2^dim] with the
+ * last axis fastest, dx[dim], iso-value v0.  counts[ncells]; centroids and
+ * normals [ncells][10][dim]. */
+int gvh_cornelius_cells(int dim, long long ncells, const double *corners, const double *dx,
+                        double v0, int *counts, double *centroids, double *normals);
+
+/* -------------------------------------------------------------- run loop */
+typedef struct gvh_run_options {
+  int freq;      /* snapshot cadence FREQ (HydroPlugin.cpp:39); reference 10   */
+  int fofreq;    /* freeze-out window FOFREQ (:40); reference 10               */
+  int fo_flags;  /* GVH_FO_*; reference 0                                      */
+  int snapshots; /* write the FREQ snapshots (outputDynamicalQuantities, :44) */
+  int surface;   /* write <root_dir>/output/surface.dat                        */
+  int verbose;   /* progress lines on stdout                                   */
+} gvh_run_options;
+
+typedef struct gvh_run_stats {
+  int steps;              /* KT steps taken                                 */
+  int frozen;             /* 1 if ended by the all-cells-frozen rule        */
+  double t_final;         /* proper time at the end                         */
+  long long surface_rows; /* rows written to surface.dat                    */
+  double ms_steps;        /* wall time in the step calls                    */
+  double ms_freezeout;    /* wall time recording and searching the surface  */
+  double ms_total;
+} gvh_run_stats;
+
+void gvh_run_options_default(gvh_run_options *ro);
+
+/* One hydrodynamic event: run() of HydroPlugin.cpp:107-434.  Initial
+ * condition types 10 (input/e.dat, ut.dat, ... under root_dir; p from the
+ * EoS) and 11 (input/Tmunu.dat) are read (InitialConditions.cpp:29-489);
+ * snapshots go to output_dir as <name>_<t>.dat (FileIO.cpp:15-60), the
+ * surface to <root_dir>/output/surface.dat (the reference opens
+ * "output/surface.dat" in the working directory, which its driver passes as
+ * the root directory).  ro and stats may be NULL. */
+int gvh_run(const gvh_lattice_parameters *lattice, const gvh_ic_parameters *ic,
+            const gvh_hydro_parameters *hydro, const char *root_dir, const char *output_dir,
+            const gvh_options *opts, const gvh_run_options *ro, gvh_run_stats *stats);
+
+/* HYDRO::run_hydro (HydroWrapper.cpp:17-87): options -c/--config DIR,
+ * -o/--output DIR, -h/--hydro (run), -t/--test; parameters from the config
+ * directory, root directory = working directory. */
+int gvh_run_hydro(int argc, char **argv, const gvh_options *opts, const gvh_run_options *ro);
+
 const char *gvh_last_error(void);
 
 #ifdef __cplusplus

@@ -55,3 +55,31 @@ def glibc_rand_sequence(n):
     for i in range(n):
         out[i] = libc.rand() / 2147483647.0
     return out
+
+
+def example_inputs():
+    """Corner values of example_cornelius.cpp: 10000 squares (2x2), then 10000
+    cubes (2x2x2), then 10000 hypercubes (2x2x2x2), all from one rand()
+    stream, row-major."""
+    seq = glibc_rand_sequence(10000 * (4 + 8 + 16))
+    sq = seq[:40000].reshape(10000, 4)
+    cu = seq[40000:120000].reshape(10000, 8)
+    hc = seq[120000:].reshape(10000, 16)
+    return sq, cu, hc
+
+
+def format_elements(dim, counts, cen, nrm):
+    """Elements printed as example_cornelius does (ostream default, 6
+    significant digits), one line per element."""
+    lines = []
+    for i in range(len(counts)):
+        for k in range(counts[i]):
+            lines.append(" ".join("%g" % x for x in list(cen[i][k]) + list(nrm[i][k])) + " ")
+    return lines
+
+
+def degenerate_cells(dim, n, seed):
+    """Cells whose corners sit on a few levels including the iso-value 0.5
+    exactly (ties, corners on the surface, ambiguous faces)."""
+    rng = np.random.default_rng(seed)
+    return rng.choice([0.2, 0.5, 0.8, 0.5, 0.5, 0.35], size=(n, 1 << dim))

@@ -205,3 +205,8 @@ def flux(params, direction, data, t, e_prev, fwd):
     H = np.empty(15, dtype=np.float64)
     lib().kto_flux(C.byref(params), direction, _ptr(data), t, e_prev, int(fwd), _ptr(H))
     return H
+
+
+def e_of_T(params, T):
+    """equilibriumEnergyDensity(T) of the oracle's EoS."""
+    return lib().kto_e_of_T(C.byref(params), T)

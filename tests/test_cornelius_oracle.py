@@ -19,14 +19,7 @@ def fmt(x):
 
 
 def example_inputs():
-    """Corner values of example_cornelius.cpp: 10000 squares (2x2), then 10000
-    cubes (2x2x2), then 10000 hypercubes (2x2x2x2), all from one rand()
-    stream, row-major."""
-    seq = CL.glibc_rand_sequence(10000 * (4 + 8 + 16))
-    sq = seq[:40000].reshape(10000, 4)
-    cu = seq[40000:120000].reshape(10000, 8)
-    hc = seq[120000:].reshape(10000, 16)
-    return sq, cu, hc
+    return CL.example_inputs()
 
 
 @pytest.fixture(scope="module")
@@ -67,11 +60,7 @@ def test_golden_outputs(inputs, dim):
 def test_against_reference_build(inputs, dim):
     """Full precision against the reference's cornelius: the random cells,
     cells with corners exactly at the iso-value, and anisotropic spacings."""
-    rng = np.random.default_rng(10)
-    cells = list(inputs[dim - 2][:3000])
-    for _ in range(2000):
-        c = rng.choice([0.2, 0.5, 0.8, 0.5, 0.5, 0.35], size=2 ** dim)
-        cells.append(c)
+    cells = list(inputs[dim - 2][:3000]) + list(CL.degenerate_cells(dim, 2000, 10))
     dxs = [np.ones(dim), np.array([0.01, 0.1, 0.1, 0.1][:dim])]
     for dx in dxs:
         for c in cells:
