@@ -81,6 +81,10 @@ def main():
     ap.add_argument("--t0", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--eos", choices=["table", "wb", "conformal"], default="table",
+                    help="equation of state (the workload's is the 2000-node table)")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="stage-kernel arrangement (library default if unset)")
     ap.add_argument("--timed-steps", type=int, default=5,
                     help="extra steps with HIP events around each stage kernel")
     ap.add_argument("--cpu-steps", type=int, default=6)
@@ -95,6 +99,8 @@ def main():
               "one MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.variant is not None:
+        os.environ["GVH_VARIANT"] = str(args.variant)
     import torch
     import gpuvh
 
@@ -113,8 +119,8 @@ def main():
     prec = gpuvh.FP64 if args.precision == "fp64" else gpuvh.MIXED
     L = gpuvh.lattice(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, args.dt)
     H = gpuvh.hydro(args.t0, 0.16, T_fo_GeV=0.15)
-    opts = gpuvh.options(gpuvh.EOS_TABLE, True, prec, 0.16 / gpuvh.HBARC,
-                         stream=stream.cuda_stream)
+    eos = {"table": gpuvh.EOS_TABLE, "wb": gpuvh.EOS_WB, "conformal": gpuvh.EOS_CONFORMAL}[args.eos]
+    opts = gpuvh.options(eos, True, prec, 0.16 / gpuvh.HBARC, stream=stream.cuda_stream)
     ctx = gpuvh.Context(L, H, opts)
     ut, ux, uy, un, pi10, Pi = gpuvh.ic.at_rest(e.shape)
     p = ctx.equilibrium_pressure(e)
@@ -175,10 +181,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == gpuvh.FP64 else "f32/f64 mixed",
         "data": "synthetic",
-        "config": {"workload": "3+1D KT step %dx%dx%d, tabulated EoS, shear+bulk, %s"
-                               % (args.nx, args.ny, args.nz, args.precision),
+        "config": {"workload": "3+1D KT step %dx%dx%d, %s EoS, shear+bulk, %s"
+                               % (args.nx, args.ny, args.nz,
+                                  "tabulated" if args.eos == "table" else args.eos, args.precision),
                    "lattice": [args.nx, args.ny, args.nz], "dtau": args.dt, "tau0": args.t0,
-                   "eos": "tabulated 2000 nodes", "eta_over_s": 0.16,
+                   "eos": "tabulated 2000 nodes" if args.eos == "table" else args.eos,
+                   "eta_over_s": 0.16,
                    "zeta_peak_GeV": 0.16, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

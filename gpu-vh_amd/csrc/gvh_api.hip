@@ -9,6 +9,7 @@
 // are plain copies.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -21,7 +22,8 @@
 #include "gvh_internal.h"
 
 namespace gvh {
-hipError_t launch_stage_fp64(const StageArgs &A, int eos, bool bulk, bool stage2, hipStream_t st);
+hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, bool mixed,
+                        hipStream_t st);
 hipError_t launch_count_above(const double *e, int nx, int ny, int nz, int ncx, int ncy, double thr,
                               unsigned long long *out, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t st);
@@ -252,9 +254,11 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
     }
     c->own_stream = true;
   }
-  // one allocation for all state: 3 x 15 conserved, 3 x 4 velocity, 2 x 2 (e, p)
-  const size_t nbuf = 3 * 15 + 3 * 4 + 4;
-  const size_t bytes = nbuf * (size_t)c->G * sizeof(double);
+  // one allocation for all state: 3 conserved buffers, 3 x 4 velocity,
+  // 2 x 2 (e, p); a mixed conserved buffer holds 4 G doubles + 11 G floats
+  const size_t Gs = (size_t)c->G;
+  c->qstride = c->precision == GVH_MIXED ? 4 * Gs + ((11 * Gs + 1) / 2 + 63) / 64 * 64 : 15 * Gs;
+  const size_t bytes = (3 * c->qstride + 16 * Gs) * sizeof(double);
   hipError_t e = hipMalloc(&c->arena, bytes);
   if (e != hipSuccess) {
     gvh_destroy(c);
@@ -272,8 +276,8 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
     return gvh_fail(GVH_EHIP, "context set-up: %s", hipGetErrorString(e));
   }
   double *a = (double *)c->arena;
-  for (int b = 0; b < 3; ++b) c->qbuf[b] = a + (size_t)b * 15 * c->G;
-  a += 45 * (size_t)c->G;
+  for (int b = 0; b < 3; ++b) c->qbuf[b] = a + (size_t)b * c->qstride;
+  a += 3 * c->qstride;
   for (int b = 0; b < 3; ++b) c->ubuf[b] = a + (size_t)b * 4 * c->G;
   a += 12 * (size_t)c->G;
   for (int b = 0; b < 2; ++b) {
@@ -382,7 +386,16 @@ extern "C" int gvh_set_initial_state(gvh_context *c, double t0, const double *e,
   c->iup = 1;
   c->ie = 0;
   const size_t B = sizeof(double);
-  HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq], hq.data(), 15 * G * B, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> hqf;
+  if (c->precision == GVH_MIXED) {
+    // T^{tau mu} in FP64, pi and Pi stored in FP32
+    hqf.assign(hq.begin() + 4 * G, hq.end());
+    HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq], hq.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq] + 4 * G, hqf.data(), 11 * G * sizeof(float),
+                           hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq], hq.data(), 15 * G * B, hipMemcpyHostToDevice, c->stream));
+  }
   HIP_TRY(hipMemcpyAsync(c->ebuf[c->ie], he.data(), G * B, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->pbuf[c->ie], hp.data(), G * B, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->ubuf[c->iu], hu.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
@@ -393,8 +406,18 @@ extern "C" int gvh_set_initial_state(gvh_context *c, double t0, const double *e,
   return GVH_OK;
 }
 
-static double *field_ptr(gvh_context *c, int f) {
-  if (f >= GVH_TTT && f <= GVH_PI) return c->qbuf[c->iq] + (size_t)f * c->G;
+// Device address of field f of the current state; *is_float tells a pi / Pi
+// field stored in FP32 (mixed form).
+static void *field_dev(gvh_context *c, int f, bool *is_float) {
+  *is_float = false;
+  if (f >= GVH_TTT && f <= GVH_PI) {
+    double *q = c->qbuf[c->iq];
+    if (c->precision == GVH_MIXED && f >= GVH_PITT) {
+      *is_float = true;
+      return reinterpret_cast<float *>(q + 4 * c->G) + (size_t)(f - GVH_PITT) * c->G;
+    }
+    return q + (size_t)f * c->G;
+  }
   if (f == GVH_E) return c->ebuf[c->ie];
   if (f == GVH_P) return c->pbuf[c->ie];
   if (f >= GVH_UT && f <= GVH_UN) return c->ubuf[c->iu] + (size_t)(f - GVH_UT) * c->G;
@@ -402,10 +425,33 @@ static double *field_ptr(gvh_context *c, int f) {
   return nullptr;
 }
 
+int gvh_copy_q_field(gvh_context *c, const double *qbase, int f, double *host) {
+  const size_t G = (size_t)c->G;
+  if (c->precision == GVH_MIXED && f >= GVH_PITT) {
+    std::vector<float> tmp(G);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), reinterpret_cast<const float *>(qbase + 4 * G) + (f - GVH_PITT) * G,
+                           G * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < G; ++i) host[i] = tmp[i];
+    return GVH_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(host, qbase + f * G, G * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GVH_OK;
+}
+
 extern "C" int gvh_set_field(gvh_context *c, int field, const double *host_ghosted) {
   if (!c || !host_ghosted) return gvh_fail(GVH_EINVAL, "gvh_set_field: NULL argument");
-  double *d = field_ptr(c, field);
+  bool fl;
+  void *d = field_dev(c, field, &fl);
   if (!d) return gvh_fail(GVH_EINVAL, "unknown field %d", field);
+  if (fl) {
+    std::vector<float> tmp(host_ghosted, host_ghosted + c->G);
+    HIP_TRY(hipMemcpyAsync(d, tmp.data(), (size_t)c->G * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GVH_OK;
+  }
   HIP_TRY(hipMemcpyAsync(d, host_ghosted, (size_t)c->G * sizeof(double), hipMemcpyHostToDevice,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -414,8 +460,25 @@ extern "C" int gvh_set_field(gvh_context *c, int field, const double *host_ghost
 
 extern "C" int gvh_get_field(gvh_context *c, int field, int ghosted, double *host_out) {
   if (!c || !host_out) return gvh_fail(GVH_EINVAL, "gvh_get_field: NULL argument");
-  double *d = field_ptr(c, field);
-  if (!d) return gvh_fail(GVH_EINVAL, "unknown field %d", field);
+  bool fl;
+  void *dv = field_dev(c, field, &fl);
+  if (!dv) return gvh_fail(GVH_EINVAL, "unknown field %d", field);
+  if (fl) {
+    std::vector<double> full(c->G);
+    int rc = gvh_copy_q_field(c, c->qbuf[c->iq], field, full.data());
+    if (rc) return rc;
+    if (ghosted) {
+      std::copy(full.begin(), full.end(), host_out);
+    } else {
+      for (int k = 0; k < c->nz; ++k)
+        for (int j = 0; j < c->ny; ++j)
+          for (int i = 0; i < c->nx; ++i)
+            host_out[i + (size_t)c->nx * (j + (size_t)c->ny * k)] =
+                full[(i + 2) + (size_t)c->ncx * ((j + 2) + (size_t)c->ncy * (k + 2))];
+    }
+    return GVH_OK;
+  }
+  const double *d = (const double *)dv;
   if (ghosted) {
     HIP_TRY(hipMemcpyAsync(host_out, d, (size_t)c->G * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -515,7 +578,7 @@ static int step_impl(gvh_context *c, double t, float *ms1, float *ms2) {
 }
 
 int launch_stage(gvh_context *c, const StageArgs &A, bool stage2) {
-  hipError_t e = launch_stage_fp64(A, c->eos.kind, c->bulk, stage2, c->stream);
+  hipError_t e = launch_stage(A, c->eos.kind, c->bulk, stage2, c->precision == GVH_MIXED, c->stream);
   if (e != hipSuccess) return gvh_fail(GVH_EHIP, "stage kernel launch: %s", hipGetErrorString(e));
   return GVH_OK;
 }

@@ -67,12 +67,16 @@ struct StageArgs {
 // same sign class (x < 0 versus x >= 0) and +0 otherwise -- exactly, since
 // min and the factors 2 and 1/2 do not round.  That form needs no products
 // of signs.
-__device__ __forceinline__ double slope(double x, double y, double z) {
-  const double dl = THETA * (y - x), dc = (z - x) * 0.5, dr = THETA * (z - y);
-  const double m = fmin(fabs(dl), fmin(fabs(dc), fabs(dr)));
-  const bool nl = dl < 0.0, nc = dc < 0.0, nr = dr < 0.0;
+template <typename R>
+__device__ __forceinline__ R slope_t(R x, R y, R z) {
+  const R dl = R(THETA) * (y - x), dc = (z - x) * R(0.5), dr = R(THETA) * (z - y);
+  const R m = fmin(fabs(dl), fmin(fabs(dc), fabs(dr)));
+  const bool nl = dl < R(0), nc = dc < R(0), nr = dr < R(0);
   const bool same = (nl == nc) & (nc == nr);
-  return same ? (nl ? -m : m) : 0.0;
+  return same ? (nl ? -m : m) : R(0);
+}
+__device__ __forceinline__ double slope(double x, double y, double z) {
+  return slope_t<double>(x, y, z);
 }
 
 // ------------------------------------------------ division, square root
@@ -282,6 +286,47 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
   }
 }
 
+// N independent root-finds sharing the guess ePrev, run in lockstep: every
+// iteration evaluates all N Newton steps without branches and a chain keeps
+// its iterate once it has converged, so each result equals solve_e's for
+// that chain.  A wave already iterates until its slowest lane converges, so
+// the instruction count matches N sequential solves while the N table
+// gathers and dependency chains overlap.
+template <int EOS, int N>
+__device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev, const double *M0,
+                                                 const double *M, const double *Pi, double *e,
+                                                 unsigned long long *diag) {
+  if constexpr (EOS == EOS_CONFORMAL) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) e[k] = fabs(sqrt(fabs(4.0 * M0[k] * M0[k] - 3.0 * M[k])) - M0[k]);
+  } else {
+    unsigned act = (1u << N) - 1u;
+#pragma unroll
+    for (int k = 0; k < N; ++k) e[k] = ePrev;
+    for (int it = 0; it < MAX_ITERS / 2 && act; ++it) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        double ek = e[k];
+        const bool done = newton_step<EOS, false>(T, ek, M0[k], M[k], Pi[k]);
+        const bool live = (act >> k) & 1u;
+        e[k] = live ? ek : e[k];
+        if (live && done) act &= ~(1u << k);
+      }
+    }
+    for (int it = MAX_ITERS / 2; it < MAX_ITERS && act; ++it) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        double ek = e[k];
+        const bool done = newton_step<EOS, true>(T, ek, M0[k], M[k], Pi[k]);
+        const bool live = (act >> k) & 1u;
+        e[k] = live ? ek : e[k];
+        if (live && done) act &= ~(1u << k);
+      }
+    }
+    if (act) atomicAdd(&diag[DIAG_NOT_CONVERGED], (unsigned long long)__popc(act));
+  }
+}
+
 struct Prim {
   double e, p, ut, ux, uy, un;
 };
@@ -340,6 +385,44 @@ __device__ __forceinline__ double flux_velocity(const EosTable &T, double t, dou
   const double P = p + Pi;
   const double Md = DIR == 0 ? M1 : (DIR == 1 ? M2 : M3);
   return ((e + P > 0.0) ? Md : -Md) / fabs(M0 + P);
+}
+
+// flux_velocity of N half-site states of one direction that share the
+// guess ePrev, with their root-finds in lockstep (solve_e_lockstep).
+// W[k] = (T^tt, T^tx, T^ty, T^tn, pi^tt, pi^tx, pi^ty, pi^tn, Pi).
+template <int EOS, int DIR, int N, typename R>
+__device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double t,
+                                                       const R (*W)[9], double ePrev,
+                                                       unsigned long long *diag, double *v) {
+  double M0[N], M[N], Md[N], Pi[N], e[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    M0[k] = (double)W[k][0] - (double)W[k][4];
+    const double M1 = (double)W[k][1] - (double)W[k][5], M2 = (double)W[k][2] - (double)W[k][6],
+                 M3 = (double)W[k][3] - (double)W[k][7];
+    M[k] = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+    Md[k] = DIR == 0 ? M1 : (DIR == 1 ? M2 : M3);
+    Pi[k] = (double)W[k][8];
+  }
+  if (ePrev <= 0.1) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) e[k] = M0[k] - M[k] / M0[k];
+  } else {
+    solve_e_lockstep<EOS, N>(T, ePrev, M0, M, Pi, e, diag);
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double ek = e[k];
+    if (isnan(ek)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
+    double p = eos_p<EOS>(T, ek);
+    if (ek < 1.e-7) {
+      ek = 1.e-7;
+      p = 1.e-7;
+      atomicAdd(&diag[DIAG_E_CLAMPED], 1ull);
+    }
+    const double P = p + Pi[k];
+    v[k] = ((ek + P > 0.0) ? Md[k] : -Md[k]) / fabs(M0[k] + P);
+  }
 }
 
 }  // namespace gvh

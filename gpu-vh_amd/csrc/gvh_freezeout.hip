@@ -191,9 +191,11 @@ hipError_t scan_with_total(const unsigned *in, unsigned *out, size_t n, unsigned
 }
 
 // ---------------------------------------------------------------- record
+// pi and Pi are read from qP: 11 blocks of G values of the storage type PS
+template <typename PS>
 __global__ __launch_bounds__(256) void fo_record_kernel(const double *__restrict__ e,
                                                         const double *__restrict__ u,
-                                                        const double *__restrict__ q, long long G,
+                                                        const PS *__restrict__ qP, long long G,
                                                         int nx, int ny, int nz, int ncx, int ncy,
                                                         int bulk, double *__restrict__ slot,
                                                         long long nint) {
@@ -203,16 +205,17 @@ __global__ __launch_bounds__(256) void fo_record_kernel(const double *__restrict
   const long long r = c / nx;
   const int j = (int)(r % ny), k = (int)(r / ny);
   const long long s = (i + 2) + (long long)ncx * ((j + 2) + (long long)ncy * (k + 2));
+  auto pi = [&](int f) { return (double)qP[(f - GVH_PITT) * G + s]; };
   slot[0 * nint + c] = u[1 * G + s];
   slot[1 * nint + c] = u[2 * G + s];
   slot[2 * nint + c] = u[3 * G + s];
   slot[3 * nint + c] = e[s];
-  slot[4 * nint + c] = q[GVH_PIXX * G + s];
-  slot[5 * nint + c] = q[GVH_PIXY * G + s];
-  slot[6 * nint + c] = q[GVH_PIXN * G + s];
-  slot[7 * nint + c] = q[GVH_PIYY * G + s];
-  slot[8 * nint + c] = q[GVH_PIYN * G + s];
-  slot[9 * nint + c] = bulk ? q[GVH_PI * G + s] : 0.0;
+  slot[4 * nint + c] = pi(GVH_PIXX);
+  slot[5 * nint + c] = pi(GVH_PIXY);
+  slot[6 * nint + c] = pi(GVH_PIXN);
+  slot[7 * nint + c] = pi(GVH_PIYY);
+  slot[8 * nint + c] = pi(GVH_PIYN);
+  slot[9 * nint + c] = bulk ? pi(GVH_PI) : 0.0;
 }
 
 // ------------------------------------------------------- window geometry
@@ -614,8 +617,14 @@ extern "C" int gvh_freezeout_record(gvh_context *c, int n) {
   const double *u = c->ubuf[stale ? c->iup : c->iu];
   const double *q = c->qbuf[stale ? c->iQ() : c->iq];
   double *dst = f->ring + (size_t)f->slot[logical] * NVAR * f->nint;
-  fo_record_kernel<<<grid_for(f->nint, 256), 256, 0, c->stream>>>(
-      c->ebuf[c->ie], u, q, c->G, c->nx, c->ny, c->nz, c->ncx, c->ncy, c->bulk ? 1 : 0, dst, f->nint);
+  if (c->precision == GVH_MIXED)
+    fo_record_kernel<float><<<grid_for(f->nint, 256), 256, 0, c->stream>>>(
+        c->ebuf[c->ie], u, reinterpret_cast<const float *>(q + 4 * c->G), c->G, c->nx, c->ny, c->nz,
+        c->ncx, c->ncy, c->bulk ? 1 : 0, dst, f->nint);
+  else
+    fo_record_kernel<double><<<grid_for(f->nint, 256), 256, 0, c->stream>>>(
+        c->ebuf[c->ie], u, q + 4 * c->G, c->G, c->nx, c->ny, c->nz, c->ncx, c->ncy,
+        c->bulk ? 1 : 0, dst, f->nint);
   HIP_TRY(hipGetLastError());
   return GVH_OK;
 }

@@ -30,6 +30,11 @@ void fo_free(FoState *f);
 
 int gvh_fail(int code, const char *fmt, ...);
 
+struct gvh_context;
+// Conserved field f (GVH_TTT..GVH_PI) of conserved buffer qbase as doubles on
+// the full ghosted grid, whatever its storage precision.
+int gvh_copy_q_field(gvh_context *c, const double *qbase, int f, double *host_ghosted);
+
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t _e = (expr);                                                        \
@@ -55,7 +60,10 @@ struct gvh_context {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
   void *arena = nullptr;
+  // conserved buffers: T^{tau mu} as 4 blocks of G doubles, then pi and Pi as
+  // 11 blocks of G doubles (FP64) or floats (mixed); qstride doubles apart
   double *qbuf[3] = {nullptr, nullptr, nullptr};  // [0],[1]: q^n / Q (swap), [2]: qS
+  size_t qstride = 0;
   double *ubuf[3] = {nullptr, nullptr, nullptr};  // u^n, u^{n-1}, u* (rotate)
   double *ebuf[2] = {nullptr, nullptr};
   double *pbuf[2] = {nullptr, nullptr};

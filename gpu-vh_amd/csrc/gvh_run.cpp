@@ -142,25 +142,34 @@ int write_snapshot(gvh_context *c, double t, const std::string &outdir, int flag
   const bool stale = stale_view(c, flags);
   const double *ud = c->ubuf[stale ? c->iup : c->iu];
   const double *qd = c->qbuf[stale ? c->iQ() : c->iq];
+  // u from the device buffer the host view reads; conserved fields by id
   struct Item {
     const char *name;
-    const double *dev;
+    const double *dev;  // velocity / e / p block, or nullptr for a conserved field
+    int field;
   };
-  std::vector<Item> items = {{"e", c->ebuf[c->ie]},  {"p", c->pbuf[c->ie]},  {"ux", ud + 1 * G},
-                             {"uy", ud + 2 * G},     {"un", ud + 3 * G},     {"ut", ud + 0 * G},
-                             {"pixx", qd + GVH_PIXX * G}, {"pixy", qd + GVH_PIXY * G},
-                             {"pixn", qd + GVH_PIXN * G}, {"piyy", qd + GVH_PIYY * G},
-                             {"piyn", qd + GVH_PIYN * G}, {"pitt", qd + GVH_PITT * G},
-                             {"pitx", qd + GVH_PITX * G}, {"pity", qd + GVH_PITY * G},
-                             {"pitn", qd + GVH_PITN * G}, {"pinn", qd + GVH_PINN * G}};
-  if (c->bulk) items.push_back({"Pi", qd + GVH_PI * G});
+  std::vector<Item> items = {{"e", c->ebuf[c->ie], -1},   {"p", c->pbuf[c->ie], -1},
+                             {"ux", ud + 1 * G, -1},       {"uy", ud + 2 * G, -1},
+                             {"un", ud + 3 * G, -1},       {"ut", ud + 0 * G, -1},
+                             {"pixx", nullptr, GVH_PIXX}, {"pixy", nullptr, GVH_PIXY},
+                             {"pixn", nullptr, GVH_PIXN}, {"piyy", nullptr, GVH_PIYY},
+                             {"piyn", nullptr, GVH_PIYN}, {"pitt", nullptr, GVH_PITT},
+                             {"pitx", nullptr, GVH_PITX}, {"pity", nullptr, GVH_PITY},
+                             {"pitn", nullptr, GVH_PITN}, {"pinn", nullptr, GVH_PINN}};
+  if (c->bulk) items.push_back({"Pi", nullptr, GVH_PI});
   std::vector<double> h(G);
   const int nx = c->nx, ny = c->ny, nz = c->nz, ncx = c->ncx, ncy = c->ncy;
   const double dx = c->L.latticeSpacingX, dy = c->L.latticeSpacingY,
                dz = c->L.latticeSpacingRapidity;
   for (const Item &it : items) {
-    HIP_TRY(hipMemcpyAsync(h.data(), it.dev, G * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (it.dev) {
+      HIP_TRY(hipMemcpyAsync(h.data(), it.dev, G * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+      int rc = gvh_copy_q_field(c, qd, it.field, h.data());
+      if (rc) return rc;
+    }
     char fname[1024];
     snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", outdir.c_str(), it.name, t);
     File F;

@@ -23,212 +23,213 @@
 namespace gvh {
 
 // ------------------------------------------------------ source terms
-struct Grad {
-  double ut[3], ux[3], uy[3], un[3], p[3];  // [x, y, eta] derivatives
+template <typename R>
+struct GradT {
+  R ut[3], ux[3], uy[3], un[3], p[3];  // [x, y, eta] derivatives
 };
+using Grad = GradT<double>;
 
 // loadSourceTerms2 (SourceTerms.cu:570-755) with setPimunuSourceTerms
 // (:30-256): T^{tau mu} geometric and pressure-gradient sources and the
 // relaxation right-hand sides of pi^{mu nu} and Pi.
-template <int EOS, bool BULK>
-__device__ __forceinline__ void source_terms(const StageArgs &A, const double *Q,
-                                             double e, double p, double ut,
-                                             double ux, double uy, double un,
-                                             double utp, double uxp, double uyp,
-                                             double unp, const Grad &g, double *S) {
-  const double t = A.t;
-  const double inv_t = 1.0 / t;
-  const double t2 = t * t, t3 = t * t2;
-  const double inv_t2 = 1.0 / t2;
-  const double Pi = BULK ? Q[14] : 0.0;
-  const double pitt = Q[4], pitx = Q[5], pity = Q[6], pitn = Q[7], pixx = Q[8],
+template <int EOS, bool BULK, typename R>
+__device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, double e_eos,
+                                             R e, R p, R ut, R ux, R uy, R un, R utp, R uxp,
+                                             R uyp, R unp, const GradT<R> &g, R *S) {
+  const R t = R(A.t);
+  const R inv_t = R(1.0) / t;
+  const R t2 = t * t, t3 = t * t2;
+  const R inv_t2 = R(1.0) / t2;
+  const R Pi = BULK ? Q[14] : R(0.0);
+  const R pitt = Q[4], pitx = Q[5], pity = Q[6], pitn = Q[7], pixx = Q[8],
                pixy = Q[9], pixn = Q[10], piyy = Q[11], piyn = Q[12], pinn = Q[13];
-  const double inv_ut = 1.0 / ut;
+  const R inv_ut = R(1.0) / ut;
 
   // T^{tau mu} sources, SourceTerms.cu:726-737
-  const double PP = p + Pi;
-  const double tnn = (e + PP) * un * un + PP * inv_t2 + pinn;
-  const double vx = ux * inv_ut, vy = uy * inv_ut, vn = un * inv_ut;
-  const double dkvk = (g.ux[0] - vx * g.ut[0]) * inv_ut + (g.uy[1] - vy * g.ut[1]) * inv_ut +
+  const R PP = p + Pi;
+  const R tnn = (e + PP) * un * un + PP * inv_t2 + pinn;
+  const R vx = ux * inv_ut, vy = uy * inv_ut, vn = un * inv_ut;
+  const R dkvk = (g.ux[0] - vx * g.ut[0]) * inv_ut + (g.uy[1] - vy * g.ut[1]) * inv_ut +
                       (g.un[2] - vn * g.ut[2]) * inv_ut;
   S[0] = -(Q[0] * inv_t + t * tnn) + dkvk * (pitt - p - Pi) - vx * g.p[0] - vy * g.p[1] -
          vn * g.p[2];
   S[1] = -Q[1] * inv_t - g.p[0] + dkvk * pitx;
   S[2] = -Q[2] * inv_t - g.p[1] + dkvk * pity;
-  S[3] = -3.0 * Q[3] * inv_t - g.p[2] * inv_t2 + dkvk * pitn;
+  S[3] = -R(3.0) * Q[3] * inv_t - g.p[2] * inv_t2 + dkvk * pitn;
 
   // transport coefficients, SourceTerms.cu:46-60
-  const double T = eos_T<EOS>(A.tab, e);
-  const double taupiInv = (double)0.2f * (T / A.etabar);
-  const double beta_pi = (double)0.2f * (e + p);
+  const double T_eos = eos_T<EOS>(A.tab, e_eos);
+  const R T = R(T_eos);
+  const R taupiInv = R(0.2f) * (T / R(A.etabar));
+  const R beta_pi = R(0.2f) * (e + p);
 
   // backward-difference time derivative and covariant acceleration, :67-86
-  const double dtut = (ut - utp) * A.inv_dt, dtux = (ux - uxp) * A.inv_dt;
-  const double dtuy = (uy - uyp) * A.inv_dt, dtun = (un - unp) * A.inv_dt;
-  const double un2 = un * un, ut2 = ut * ut;
-  const double dut = ut * dtut + ux * g.ut[0] + uy * g.ut[1] + un * g.ut[2];
-  const double Dut = dut + t * un2;
-  const double dux = ut * dtux + ux * g.ux[0] + uy * g.ux[1] + un * g.ux[2];
-  const double duy = ut * dtuy + ux * g.uy[0] + uy * g.uy[1] + un * g.uy[2];
-  const double dun = ut * dtun + ux * g.un[0] + uy * g.un[1] + un * g.un[2];
-  const double Dux = -dux, Duy = -duy;
-  const double Dun = -t2 * dun - 2.0 * t * ut * un;
+  const R dtut = (ut - utp) * R(A.inv_dt), dtux = (ux - uxp) * R(A.inv_dt);
+  const R dtuy = (uy - uyp) * R(A.inv_dt), dtun = (un - unp) * R(A.inv_dt);
+  const R un2 = un * un, ut2 = ut * ut;
+  const R dut = ut * dtut + ux * g.ut[0] + uy * g.ut[1] + un * g.ut[2];
+  const R Dut = dut + t * un2;
+  const R dux = ut * dtux + ux * g.ux[0] + uy * g.ux[1] + un * g.ux[2];
+  const R duy = ut * dtuy + ux * g.uy[0] + uy * g.uy[1] + un * g.uy[2];
+  const R dun = ut * dtun + ux * g.un[0] + uy * g.un[1] + un * g.un[2];
+  const R Dux = -dux, Duy = -duy;
+  const R Dun = -t2 * dun - R(2.0) * t * ut * un;
 
-  const double theta = ut * inv_t + dtut + g.ux[0] + g.uy[1] + g.un[2];
-  const double th3 = theta * (1.0 / 3.0);
+  const R theta = ut * inv_t + dtut + g.ux[0] + g.uy[1] + g.un[2];
+  const R th3 = theta * (R(1.0) / R(3.0));
 
   // sigma^{mu nu}, :96-106
-  const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1.0) * th3;
-  const double stx = -(t * un2 * ux) * 0.5 + (dtux - g.ut[0]) * 0.5 -
-                     (ux * dut + ut * dux) * 0.5 + ut * ux * th3;
-  const double sty = -(t * un2 * uy) * 0.5 + (dtuy - g.ut[1]) * 0.5 -
-                     (uy * dut + ut * duy) * 0.5 + ut * uy * th3;
-  const double stn = -un * (2.0 * ut2 + t2 * un2) * (0.5 * inv_t) +
-                     (dtun - g.ut[2] * inv_t2) * 0.5 - (un * dut + ut * dun) * 0.5 +
+  const R stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - R(1.0)) * th3;
+  const R stx = -(t * un2 * ux) * R(0.5) + (dtux - g.ut[0]) * R(0.5) -
+                     (ux * dut + ut * dux) * R(0.5) + ut * ux * th3;
+  const R sty = -(t * un2 * uy) * R(0.5) + (dtuy - g.ut[1]) * R(0.5) -
+                     (uy * dut + ut * duy) * R(0.5) + ut * uy * th3;
+  const R stn = -un * (R(2.0) * ut2 + t2 * un2) * (R(0.5) * inv_t) +
+                     (dtun - g.ut[2] * inv_t2) * R(0.5) - (un * dut + ut * dun) * R(0.5) +
                      ut * un * th3;
-  const double sxx = -(g.ux[0] + ux * dux) + (1.0 + ux * ux) * th3;
-  const double sxy = -(g.uy[0] + g.ux[1]) * 0.5 - (uy * dux + ux * duy) * 0.5 + ux * uy * th3;
-  const double sxn = -ut * ux * un * inv_t - (g.un[0] + g.ux[2] * inv_t2) * 0.5 -
-                     (un * dux + ux * dun) * 0.5 + ux * un * th3;
-  const double syy = -(g.uy[1] + uy * duy) + (1.0 + uy * uy) * th3;
-  const double syn = -ut * uy * un * inv_t - (g.un[1] + g.uy[2] * inv_t2) * 0.5 -
-                     (un * duy + uy * dun) * 0.5 + uy * un * th3;
-  const double snn = -ut * (1.0 + 2.0 * t2 * un2) / t3 - g.un[2] * inv_t2 - un * dun +
+  const R sxx = -(g.ux[0] + ux * dux) + (R(1.0) + ux * ux) * th3;
+  const R sxy = -(g.uy[0] + g.ux[1]) * R(0.5) - (uy * dux + ux * duy) * R(0.5) + ux * uy * th3;
+  const R sxn = -ut * ux * un * inv_t - (g.un[0] + g.ux[2] * inv_t2) * R(0.5) -
+                     (un * dux + ux * dun) * R(0.5) + ux * un * th3;
+  const R syy = -(g.uy[1] + uy * duy) + (R(1.0) + uy * uy) * th3;
+  const R syn = -ut * uy * un * inv_t - (g.un[1] + g.uy[2] * inv_t2) * R(0.5) -
+                     (un * duy + uy * dun) * R(0.5) + uy * un * th3;
+  const R snn = -ut * (R(1.0) + R(2.0) * t2 * un2) / t3 - g.un[2] * inv_t2 - un * dun +
                      (inv_t2 + un2) * th3;
 
   // omega^mu_nu, :123-136
-  const double wtx = (-dtux - g.ut[0] + ut * dux - ux * dut - t * un2 * ux) * 0.5;
-  const double wty = (-dtuy - g.ut[1] + ut * duy - uy * dut - t * un2 * uy) * 0.5;
-  const double wtn = (-t2 * dtun - 2.0 * t * un - g.ut[2] - ut * Dun - t2 * un * dut -
-                      t3 * un * un2) * 0.5;
-  const double wxy = (g.uy[0] - g.ux[1] + ux * duy - uy * dux) * 0.5;
-  const double wxn = (t2 * g.un[0] - g.ux[2] - ux * Dun - t2 * un * dux) * 0.5;
-  const double wyn = (t2 * g.un[1] - g.uy[2] - uy * Dun - t2 * un * duy) * 0.5;
-  const double wxt = wtx, wyt = wty, wnt = wtn * inv_t2, wyx = -wxy;
-  const double wnx = -wxn * inv_t2, wny = -wyn * inv_t2;
+  const R wtx = (-dtux - g.ut[0] + ut * dux - ux * dut - t * un2 * ux) * R(0.5);
+  const R wty = (-dtuy - g.ut[1] + ut * duy - uy * dut - t * un2 * uy) * R(0.5);
+  const R wtn = (-t2 * dtun - R(2.0) * t * un - g.ut[2] - ut * Dun - t2 * un * dut -
+                      t3 * un * un2) * R(0.5);
+  const R wxy = (g.uy[0] - g.ux[1] + ux * duy - uy * dux) * R(0.5);
+  const R wxn = (t2 * g.un[0] - g.ux[2] - ux * Dun - t2 * un * dux) * R(0.5);
+  const R wyn = (t2 * g.un[1] - g.uy[2] - uy * Dun - t2 * un * duy) * R(0.5);
+  const R wxt = wtx, wyt = wty, wnt = wtn * inv_t2, wyx = -wxy;
+  const R wnx = -wxn * inv_t2, wny = -wyn * inv_t2;
 
   // pi^{mu lambda} D u_lambda for each row mu, then I1 = u^mu a^nu + u^nu a^mu
-  const double at = pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun;
-  const double ax = pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun;
-  const double ay = pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun;
-  const double an = pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun;
+  const R at = pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun;
+  const R ax = pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun;
+  const R ay = pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun;
+  const R an = pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun;
 
   // pi:sigma, :185-187
-  const double ps = pitt * stt - 2.0 * pitx * stx - 2.0 * pity * sty + pixx * sxx +
-                    2.0 * pixy * sxy + piyy * syy - 2.0 * pitn * stn * t2 +
-                    2.0 * pixn * sxn * t2 + 2.0 * piyn * syn * t2 + pinn * snn * t2 * t2;
-  const double ps3 = ps * (1.0 / 3.0);
+  const R ps = pitt * stt - R(2.0) * pitx * stx - R(2.0) * pity * sty + pixx * sxx +
+                    R(2.0) * pixy * sxy + piyy * syy - R(2.0) * pitn * stn * t2 +
+                    R(2.0) * pixn * sxn * t2 + R(2.0) * piyn * syn * t2 + pinn * snn * t2 * t2;
+  const R ps3 = ps * (R(1.0) / R(3.0));
 
-  constexpr double d_pp = 1.33333, t_pp = 1.42857, l_pP = 1.2;  // TransportCoefficients.cuh:14-16
-  const double lPi = l_pP * Pi;
+  constexpr R d_pp = R(1.33333), t_pp = R(1.42857), l_pP = R(1.2);  // TransportCoefficients.cuh:14-16
+  const R lPi = l_pP * Pi;
 
   // I = I1 + delta I2 - I3 + tau I4 - lambda Pi sigma, relaxation RHS with
   // Milne Christoffel terms, :141-233; RHS/u^tau + pi dkvk, :243-252
-  const double b2 = 2.0 * beta_pi;
+  const R b2 = R(2.0) * beta_pi;
   {
-    const double I1 = 2.0 * ut * at;
-    const double I3 = 2.0 * (pitx * wtx + pity * wty + pitn * wtn);
-    const double I4 = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - (1.0 - ut2) * ps3;
-    const double I = I1 + d_pp * theta * pitt - I3 + t_pp * I4 - lPi * stt;
-    const double r = b2 * stt - pitt * taupiInv - I - 2.0 * un * t * pitn;
+    const R I1 = R(2.0) * ut * at;
+    const R I3 = R(2.0) * (pitx * wtx + pity * wty + pitn * wtn);
+    const R I4 = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - (R(1.0) - ut2) * ps3;
+    const R I = I1 + d_pp * theta * pitt - I3 + t_pp * I4 - lPi * stt;
+    const R r = b2 * stt - pitt * taupiInv - I - R(2.0) * un * t * pitn;
     S[4] = r * inv_ut + pitt * dkvk;
   }
   {
-    const double I1 = ux * at + ut * ax;
-    const double I3 = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
-    const double I4 = (pitt * stx + pitx * stt) * 0.5 - (pitx * sxx + pixx * stx) * 0.5 -
-                      (pity * sxy + pixy * sty) * 0.5 - t2 * (pitn * sxn + pixn * stn) * 0.5 +
+    const R I1 = ux * at + ut * ax;
+    const R I3 = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
+    const R I4 = (pitt * stx + pitx * stt) * R(0.5) - (pitx * sxx + pixx * stx) * R(0.5) -
+                      (pity * sxy + pixy * sty) * R(0.5) - t2 * (pitn * sxn + pixn * stn) * R(0.5) +
                       (ut * ux) * ps3;
-    const double I = I1 + d_pp * theta * pitx - I3 + t_pp * I4 - lPi * stx;
-    const double r = b2 * stx - pitx * taupiInv - I - un * t * pixn;
+    const R I = I1 + d_pp * theta * pitx - I3 + t_pp * I4 - lPi * stx;
+    const R r = b2 * stx - pitx * taupiInv - I - un * t * pixn;
     S[5] = r * inv_ut + pitx * dkvk;
   }
   {
-    const double I1 = uy * at + ut * ay;
-    const double I3 = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
-    const double I4 = (pitt * sty + pity * stt) * 0.5 - (pitx * sxy + pixy * stx) * 0.5 -
-                      (pity * syy + piyy * sty) * 0.5 - t2 * (pitn * syn + piyn * stn) * 0.5 +
+    const R I1 = uy * at + ut * ay;
+    const R I3 = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
+    const R I4 = (pitt * sty + pity * stt) * R(0.5) - (pitx * sxy + pixy * stx) * R(0.5) -
+                      (pity * syy + piyy * sty) * R(0.5) - t2 * (pitn * syn + piyn * stn) * R(0.5) +
                       (ut * uy) * ps3;
-    const double I = I1 + d_pp * theta * pity - I3 + t_pp * I4 - lPi * sty;
-    const double r = b2 * sty - pity * taupiInv - I - un * t * piyn;
+    const R I = I1 + d_pp * theta * pity - I3 + t_pp * I4 - lPi * sty;
+    const R r = b2 * sty - pity * taupiInv - I - un * t * piyn;
     S[6] = r * inv_ut + pity * dkvk;
   }
   {
-    const double I1 = un * at + ut * an;
-    const double I3 = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
-    const double I4 = (pitt * stn + pitn * stt) * 0.5 - (pitx * sxn + pixn * stx) * 0.5 -
-                      (pity * syn + piyn * sty) * 0.5 - t2 * (pitn * snn + pinn * stn) * 0.5 +
+    const R I1 = un * at + ut * an;
+    const R I3 = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
+    const R I4 = (pitt * stn + pitn * stt) * R(0.5) - (pitx * sxn + pixn * stx) * R(0.5) -
+                      (pity * syn + piyn * sty) * R(0.5) - t2 * (pitn * snn + pinn * stn) * R(0.5) +
                       (ut * un) * ps3;
-    const double I = I1 + d_pp * theta * pitn - I3 + t_pp * I4 - lPi * stn;
-    const double r = b2 * stn - pitn * taupiInv - I - un * t * pinn -
+    const R I = I1 + d_pp * theta * pitn - I3 + t_pp * I4 - lPi * stn;
+    const R r = b2 * stn - pitn * taupiInv - I - un * t * pinn -
                      (ut * pitn + un * pitt) * inv_t;
     S[7] = r * inv_ut + pitn * dkvk;
   }
   {
-    const double I1 = 2.0 * ux * ax;
-    const double I3 = 2.0 * (pitx * wxt + pixy * wxy + pixn * wxn);
-    const double I4 = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + (1.0 + ux * ux) * ps3;
-    const double I = I1 + d_pp * theta * pixx - I3 + t_pp * I4 - lPi * sxx;
-    const double r = b2 * sxx - pixx * taupiInv - I;
+    const R I1 = R(2.0) * ux * ax;
+    const R I3 = R(2.0) * (pitx * wxt + pixy * wxy + pixn * wxn);
+    const R I4 = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + (R(1.0) + ux * ux) * ps3;
+    const R I = I1 + d_pp * theta * pixx - I3 + t_pp * I4 - lPi * sxx;
+    const R r = b2 * sxx - pixx * taupiInv - I;
     S[8] = r * inv_ut + pixx * dkvk;
   }
   {
-    const double I1 = uy * ax + ux * ay;
-    const double I3 = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
-    const double I4 = (pitx * sty + pity * stx) * 0.5 - (pixx * sxy + pixy * sxx) * 0.5 -
-                      (pixy * syy + piyy * sxy) * 0.5 - t2 * (pixn * syn + piyn * sxn) * 0.5 +
+    const R I1 = uy * ax + ux * ay;
+    const R I3 = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
+    const R I4 = (pitx * sty + pity * stx) * R(0.5) - (pixx * sxy + pixy * sxx) * R(0.5) -
+                      (pixy * syy + piyy * sxy) * R(0.5) - t2 * (pixn * syn + piyn * sxn) * R(0.5) +
                       (ux * uy) * ps3;
-    const double I = I1 + d_pp * theta * pixy - I3 + t_pp * I4 - lPi * sxy;
-    const double r = b2 * sxy - pixy * taupiInv - I;
+    const R I = I1 + d_pp * theta * pixy - I3 + t_pp * I4 - lPi * sxy;
+    const R r = b2 * sxy - pixy * taupiInv - I;
     S[9] = r * inv_ut + pixy * dkvk;
   }
   {
-    const double I1 = un * ax + ux * an;
-    const double I3 = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
-    const double I4 = (pitx * stn + pitn * stx) * 0.5 - (pixx * sxn + pixn * sxx) * 0.5 -
-                      (pixy * syn + piyn * sxy) * 0.5 - t2 * (pixn * snn + pinn * sxn) * 0.5 +
+    const R I1 = un * ax + ux * an;
+    const R I3 = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
+    const R I4 = (pitx * stn + pitn * stx) * R(0.5) - (pixx * sxn + pixn * sxx) * R(0.5) -
+                      (pixy * syn + piyn * sxy) * R(0.5) - t2 * (pixn * snn + pinn * sxn) * R(0.5) +
                       (ux * un) * ps3;
-    const double I = I1 + d_pp * theta * pixn - I3 + t_pp * I4 - lPi * sxn;
-    const double r = b2 * sxn - pixn * taupiInv - I - (ut * pixn + un * pitx) * inv_t;
+    const R I = I1 + d_pp * theta * pixn - I3 + t_pp * I4 - lPi * sxn;
+    const R r = b2 * sxn - pixn * taupiInv - I - (ut * pixn + un * pitx) * inv_t;
     S[10] = r * inv_ut + pixn * dkvk;
   }
   {
-    const double I1 = 2.0 * uy * ay;
-    const double I3 = 2.0 * (pity * wyt + pixy * wyx + piyn * wyn);
-    const double I4 = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + (1.0 + uy * uy) * ps3;
-    const double I = I1 + d_pp * theta * piyy - I3 + t_pp * I4 - lPi * syy;
-    const double r = b2 * syy - piyy * taupiInv - I;
+    const R I1 = R(2.0) * uy * ay;
+    const R I3 = R(2.0) * (pity * wyt + pixy * wyx + piyn * wyn);
+    const R I4 = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + (R(1.0) + uy * uy) * ps3;
+    const R I = I1 + d_pp * theta * piyy - I3 + t_pp * I4 - lPi * syy;
+    const R r = b2 * syy - piyy * taupiInv - I;
     S[11] = r * inv_ut + piyy * dkvk;
   }
   {
-    const double I1 = un * ay + uy * an;
-    const double I3 = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
-    const double I4 = (pity * stn + pitn * sty) * 0.5 - (pixy * sxn + pixn * sxy) * 0.5 -
-                      (piyy * syn + piyn * syy) * 0.5 - t2 * (piyn * snn + pinn * syn) * 0.5 +
+    const R I1 = un * ay + uy * an;
+    const R I3 = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
+    const R I4 = (pity * stn + pitn * sty) * R(0.5) - (pixy * sxn + pixn * sxy) * R(0.5) -
+                      (piyy * syn + piyn * syy) * R(0.5) - t2 * (piyn * snn + pinn * syn) * R(0.5) +
                       (uy * un) * ps3;
-    const double I = I1 + d_pp * theta * piyn - I3 + t_pp * I4 - lPi * syn;
-    const double r = b2 * syn - piyn * taupiInv - I - (ut * piyn + un * pity) * inv_t;
+    const R I = I1 + d_pp * theta * piyn - I3 + t_pp * I4 - lPi * syn;
+    const R r = b2 * syn - piyn * taupiInv - I - (ut * piyn + un * pity) * inv_t;
     S[12] = r * inv_ut + piyn * dkvk;
   }
   {
-    const double I1 = 2.0 * un * an;
-    const double I3 = 2.0 * (pitn * wnt + pixn * wnx + piyn * wny);
-    const double I4 = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + (inv_t2 + un2) * ps3;
-    const double I = I1 + d_pp * theta * pinn - I3 + t_pp * I4 - lPi * snn;
-    const double r = b2 * snn - pinn * taupiInv - I - 2.0 * (ut * pinn + un * pitn) * inv_t;
+    const R I1 = R(2.0) * un * an;
+    const R I3 = R(2.0) * (pitn * wnt + pixn * wnx + piyn * wny);
+    const R I4 = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + (inv_t2 + un2) * ps3;
+    const R I = I1 + d_pp * theta * pinn - I3 + t_pp * I4 - lPi * snn;
+    const R r = b2 * snn - pinn * taupiInv - I - R(2.0) * (ut * pinn + un * pitn) * inv_t;
     S[13] = r * inv_ut + pinn * dkvk;
   }
   if constexpr (BULK) {
     // bulk relaxation, :53-60 and :238, :254
     double p_, cs2;
-    eos_p_cs2<EOS>(A.tab, e, p_, cs2);
-    const double a = (double)0.333333f - cs2;
-    const double a2 = a * a;
-    const double beta_Pi = 15.0 * a2 * (e + p);
-    const double lambda_Pipi = (double)1.6f * a;
-    const double tauPiInv = 15.0 * a2 * (T / zeta_over_s(T, A.zeta_tpeak));
-    constexpr double d_PP = 0.666667;  // TransportCoefficients.cuh:18
-    const double dPi = -beta_Pi * theta - Pi * tauPiInv - d_PP * Pi * theta + lambda_Pipi * ps;
+    eos_p_cs2<EOS>(A.tab, e_eos, p_, cs2);
+    const R a = R(0.333333f) - R(cs2);
+    const R a2 = a * a;
+    const R beta_Pi = R(15.0) * a2 * (e + p);
+    const R lambda_Pipi = R(1.6f) * a;
+    const R tauPiInv = R(15.0) * a2 * (T / R(zeta_over_s(T_eos, A.zeta_tpeak)));
+    constexpr R d_PP = R(0.666667);  // TransportCoefficients.cuh:18
+    const R dPi = -beta_Pi * theta - Pi * tauPiInv - d_PP * Pi * theta + lambda_Pipi * ps;
     S[14] = dPi * inv_ut + Pi * dkvk;
   }
 }
@@ -252,67 +253,82 @@ constexpr int BLOCK = 256;
 // the four solves, then every variable once more for its fluxes.
 // Every stencil load is a uniform (scalar) base plus the thread's 32-bit
 // cell offset.
-template <int EOS, bool BULK, int DIR, bool KEEP_STATES>
+template <int EOS, bool BULK, int DIR, bool KEEP_STATES, bool LOCK, typename R, typename PS>
 __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s, double ePrev,
-                                                 double v_s, double *acc) {
+                                                 R v_s, double *acc) {
   constexpr int NV = BULK ? 15 : 14;
   const long long st = DIR == 0 ? 1 : (DIR == 1 ? (long long)A.ncx : (long long)A.ncx * A.ncy);
-  const double *__restrict__ q = A.qin;
   const long long G = A.G;
+  const double *__restrict__ qT = A.qin;                                    // T^{tau mu}
+  const PS *__restrict__ qP = reinterpret_cast<const PS *>(A.qin + 4 * G);  // pi, Pi
   const double t = A.t;
-  const double hc = A.half_inv_h[DIR];
+  const R hc = R(A.half_inv_h[DIR]);
+  // conserved variable n at cell s + o (n < 4: T^{tau mu}, else pi / Pi)
+  auto ld = [&](int n, long long o) -> R {
+    return n < 4 ? R((qT + n * G + o)[s]) : R((qP + (n - 4) * G + o)[s]);
+  };
+  auto ldp = [&](int n, long long o) -> R { return R((qP + (n - 4) * G + o)[s]); };
 
   double vRf, vLf, vRb, vLb;
-  double Rf[9], Lf[9], Rb[9], Lb[9];  // used when KEEP_STATES
-  double cdv[5];
+  // the four states when KEEP_STATES: [0] right of i+1/2, [1] left of i+1/2,
+  // [2] right of i-1/2, [3] left of i-1/2
+  R Wst[4][9];
+  R cdv[5];
   if constexpr (KEEP_STATES) {
     // all four states of the nine inversion variables stay in registers
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       if (a == 8 && !BULK) {
-        Rf[8] = Lf[8] = Rb[8] = Lb[8] = cdv[4] = 0.0;
+        Wst[0][8] = Wst[1][8] = Wst[2][8] = Wst[3][8] = cdv[4] = R(0);
         continue;
       }
       const int n = a < 8 ? a : 14;
-      const double *b = q + n * G;
-      const double vm2 = (b - 2 * st)[s], vm1 = (b - st)[s], v0 = b[s], vp1 = (b + st)[s],
-                   vp2 = (b + 2 * st)[s];
-      const double D0 = slope(vm1, v0, vp1);
-      Rf[a] = vp1 - slope(v0, vp1, vp2) * 0.5;
-      Lf[a] = v0 + D0 * 0.5;
-      Rb[a] = v0 - D0 * 0.5;
-      Lb[a] = vm1 + slope(vm2, vm1, v0) * 0.5;
+      const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
+              vp2 = ld(n, 2 * st);
+      const R D0 = slope_t<R>(vm1, v0, vp1);
+      Wst[0][a] = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
+      Wst[1][a] = v0 + D0 * R(0.5);
+      Wst[2][a] = v0 - D0 * R(0.5);
+      Wst[3][a] = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
       if (a >= 4) cdv[a - 4] = (vp1 - vm1) * hc;
     }
-    vRf = flux_velocity<EOS, DIR>(A.tab, t, Rf[0], Rf[1], Rf[2], Rf[3], Rf[4], Rf[5], Rf[6],
-                                  Rf[7], Rf[8], ePrev, A.diag);
-    vLf = flux_velocity<EOS, DIR>(A.tab, t, Lf[0], Lf[1], Lf[2], Lf[3], Lf[4], Lf[5], Lf[6],
-                                  Lf[7], Lf[8], ePrev, A.diag);
-    vRb = flux_velocity<EOS, DIR>(A.tab, t, Rb[0], Rb[1], Rb[2], Rb[3], Rb[4], Rb[5], Rb[6],
-                                  Rb[7], Rb[8], ePrev, A.diag);
-    vLb = flux_velocity<EOS, DIR>(A.tab, t, Lb[0], Lb[1], Lb[2], Lb[3], Lb[4], Lb[5], Lb[6],
-                                  Lb[7], Lb[8], ePrev, A.diag);
+    double v4[4];
+    if constexpr (LOCK) {
+      flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, v4);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v4[k] = flux_velocity<EOS, DIR>(A.tab, t, (double)Wst[k][0], (double)Wst[k][1],
+                                        (double)Wst[k][2], (double)Wst[k][3], (double)Wst[k][4],
+                                        (double)Wst[k][5], (double)Wst[k][6], (double)Wst[k][7],
+                                        (double)Wst[k][8], ePrev, A.diag);
+    }
+    vRf = v4[0];
+    vLf = v4[1];
+    vRb = v4[2];
+    vLb = v4[3];
   } else {
     // the four states one after another (rolled: few live registers)
     vRf = vLf = vRb = vLb = 0.0;
 #pragma unroll 1
     for (int state = 0; state < 4; ++state) {
       const long long o = state == 0 ? st : (state == 3 ? -st : 0);
-      const double sg = (state & 1) ? 0.5 : -0.5;
-      double w[9];
+      const R sg = (state & 1) ? R(0.5) : R(-0.5);
+      R w[9];
 #pragma unroll
       for (int a = 0; a < 9; ++a) {
         if (a == 8 && !BULK) {
-          w[8] = 0.0;
+          w[8] = R(0);
           continue;
         }
         const int n = a < 8 ? a : 14;
-        const double *b = q + n * G + o;
-        const double xm = (b - st)[s], x0 = b[s], xp = (b + st)[s];
-        w[a] = x0 + slope(xm, x0, xp) * sg;
+        const R xm = ld(n, o - st), x0 = ld(n, o), xp = ld(n, o + st);
+        w[a] = x0 + slope_t<R>(xm, x0, xp) * sg;
       }
-      const double v = flux_velocity<EOS, DIR>(A.tab, t, w[0], w[1], w[2], w[3], w[4], w[5], w[6],
-                                               w[7], w[8], ePrev, A.diag);
+      const double v = flux_velocity<EOS, DIR>(A.tab, t, (double)w[0], (double)w[1], (double)w[2],
+                                               (double)w[3], (double)w[4], (double)w[5],
+                                               (double)w[6], (double)w[7], (double)w[8], ePrev,
+                                               A.diag);
       vRf = state == 0 ? v : vRf;
       vLf = state == 1 ? v : vLf;
       vRb = state == 2 ? v : vRb;
@@ -321,73 +337,74 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
 #pragma unroll
     for (int a = 4; a < 9; ++a) {
       if (a == 8 && !BULK) {
-        cdv[4] = 0.0;
+        cdv[4] = R(0);
         continue;
       }
       const int n = a < 8 ? a : 14;
-      cdv[a - 4] = ((q + n * G + st)[s] - (q + n * G - st)[s]) * hc;
+      cdv[a - 4] = (ld(n, st) - ld(n, -st)) * hc;
     }
   }
   // local propagation speeds, max(|v_L|, |v_R|) (LocalPropagationSpeed.cu:16-24)
-  const double aF = fmax(fabs(vLf), fabs(vRf));
-  const double aB = fmax(fabs(vLb), fabs(vRb));
+  const R rRf = R(vRf), rLf = R(vLf), rRb = R(vRb), rLb = R(vLb);
+  const R aF = fmax(fabs(rLf), fabs(rRf));
+  const R aB = fmax(fabs(rLb), fabs(rRb));
 
   // directional sources: central differences of pi^{tau mu} and Pi and of
   // the three spatial pi components of this direction
-  auto cd = [&](int n) { return ((q + n * G + st)[s] - (q + n * G - st)[s]) * hc; };
-  const double cPi = cdv[4];
-  double Sd0, Sd1, Sd2, Sd3;
+  auto cd = [&](int n) { return (ldp(n, st) - ldp(n, -st)) * hc; };
+  const R cPi = cdv[4];
+  R Sd0, Sd1, Sd2, Sd3;
   if constexpr (DIR == 0) {
-    Sd0 = cdv[0] * v_s - cdv[1] - (BULK ? v_s * cPi : 0.0);
+    Sd0 = cdv[0] * v_s - cdv[1] - (BULK ? v_s * cPi : R(0));
     Sd1 = cdv[1] * v_s - cd(8) - cPi;
     Sd2 = cdv[2] * v_s - cd(9);
     Sd3 = cdv[3] * v_s - cd(10);
   } else if constexpr (DIR == 1) {
-    Sd0 = cdv[0] * v_s - cdv[2] - (BULK ? v_s * cPi : 0.0);
+    Sd0 = cdv[0] * v_s - cdv[2] - (BULK ? v_s * cPi : R(0));
     Sd2 = cdv[2] * v_s - cd(11) - cPi;
     Sd1 = cdv[1] * v_s - cd(9);
     Sd3 = cdv[3] * v_s - cd(12);
   } else {
-    Sd0 = cdv[0] * v_s - cdv[3] - (BULK ? v_s * cPi : 0.0);
-    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? cPi / (t * t) : 0.0);
+    Sd0 = cdv[0] * v_s - cdv[3] - (BULK ? v_s * cPi : R(0));
+    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? cPi / R(t * t) : R(0));
     Sd1 = cdv[1] * v_s - cd(10);
     Sd2 = cdv[2] * v_s - cd(12);
   }
 
-  const double ih = A.inv_h[DIR], dt = A.dt;
+  const R ih = R(A.inv_h[DIR]);
+  const double dt = A.dt;
   // H = (F(q_R) + F(q_L) - a (q_R - q_L))/2 with F = v q
 #define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
-  (((vRb * (rb) + vLb * (lb) - aB * ((rb) - (lb))) * 0.5 -                   \
-    (vRf * (rf) + vLf * (lf) - aF * ((rf) - (lf))) * 0.5) * ih)
+  (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) * R(0.5) -                \
+    (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf))) * R(0.5)) * ih)
   int n0 = 0;
   if constexpr (KEEP_STATES) {
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       if (a == 8 && !BULK) continue;
       const int n = a < 8 ? a : 14;
-      double r = GVH_FLUXDIFF(Rf[a], Lf[a], Rb[a], Lb[a]);
+      R r = GVH_FLUXDIFF(Wst[0][a], Wst[1][a], Wst[2][a], Wst[3][a]);
       if (a == 0) r += Sd0;
       if (a == 1) r += Sd1;
       if (a == 2) r += Sd2;
       if (a == 3) r += Sd3;
-      acc[n * BLOCK] += r * dt;
+      acc[n * BLOCK] += (double)r * dt;
     }
     n0 = 8;
   }
   const int n1 = KEEP_STATES ? 14 : NV;
 #pragma unroll 1
   for (int n = n0; n < n1; ++n) {
-    const double *b = q + n * G;
-    const double vm2 = (b - 2 * st)[s], vm1 = (b - st)[s], v0 = b[s], vp1 = (b + st)[s],
-                 vp2 = (b + 2 * st)[s];
-    const double D0 = slope(vm1, v0, vp1);
-    const double rf = vp1 - slope(v0, vp1, vp2) * 0.5;
-    const double lf = v0 + D0 * 0.5;
-    const double rb = v0 - D0 * 0.5;
-    const double lb = vm1 + slope(vm2, vm1, v0) * 0.5;
-    double r = GVH_FLUXDIFF(rf, lf, rb, lb);
+    const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
+            vp2 = ld(n, 2 * st);
+    const R D0 = slope_t<R>(vm1, v0, vp1);
+    const R rf = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
+    const R lf = v0 + D0 * R(0.5);
+    const R rb = v0 - D0 * R(0.5);
+    const R lb = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
+    R r = GVH_FLUXDIFF(rf, lf, rb, lb);
     if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
-    acc[n * BLOCK] += r * dt;
+    acc[n * BLOCK] += (double)r * dt;
   }
 #undef GVH_FLUXDIFF
 }
@@ -461,8 +478,8 @@ __global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
   for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? (A.qin + n * G)[s] : 0.0;
   S[14] = 0.0;
   const double *up = A.uprev;
-  source_terms<EOS, BULK>(A, Q, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s], (up + 2 * G)[s],
-                          (up + 3 * G)[s], g, S);
+  source_terms<EOS, BULK, double>(A, Q, e_s, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s],
+                                  (up + 2 * G)[s], (up + 3 * G)[s], g, S);
 #pragma unroll
   for (int n = 0; n < NV; ++n) (A.qout + n * G)[s] = Q[n] + A.dt * S[n];
 }
@@ -473,7 +490,8 @@ __global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
 // copies.  acc starts from what kt_source_kernel stored at the cell.
 // FUSED: the source terms are computed here first (one kernel per stage);
 // otherwise kt_source_kernel has stored acc = q + dt S at the cell.
-template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP>
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, bool LOCK,
+          typename R, typename PS>
 __global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
   constexpr int NV = BULK ? 15 : 14;
   __shared__ double acc_lds[15 * BLOCK];
@@ -484,49 +502,59 @@ __global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
   const unsigned s = c.s;
   const long long sy = A.ncx, sz = (long long)A.ncx * A.ncy;
   const long long G = A.G;
+  // conserved variable n of a buffer at this cell: T^{tau mu} in FP64, pi
+  // and Pi in the storage type PS
+  auto q_at = [&](const double *base, int n) -> double {
+    return n < 4 ? (base + n * G)[s]
+                 : (double)(reinterpret_cast<const PS *>(base + 4 * G) + (n - 4) * G)[s];
+  };
   const double *u = A.uin;
   const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
   const double e_s = A.ein[s];
   if constexpr (FUSED) {
     const double p_s = A.pin[s];
-    Grad g;
+    GradT<R> g;
     const long long st[3] = {1, sy, sz};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (d == 2 && !HAS_Z) {
-        g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = 0.0;
+        g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = R(0);
         continue;
       }
-      const double ih = A.inv_h[d];
+      const R ih = R(A.inv_h[d]);
       const double *um = u - st[d], *upl = u + st[d];
-      g.ut[d] = slope(um[s], ut, upl[s]) * ih;
-      g.ux[d] = slope((um + G)[s], ux, (upl + G)[s]) * ih;
-      g.uy[d] = slope((um + 2 * G)[s], uy, (upl + 2 * G)[s]) * ih;
-      g.un[d] = slope((um + 3 * G)[s], un, (upl + 3 * G)[s]) * ih;
-      g.p[d] = slope((A.pin - st[d])[s], p_s, (A.pin + st[d])[s]) * ih;
+      g.ut[d] = slope_t<R>(R(um[s]), R(ut), R(upl[s])) * ih;
+      g.ux[d] = slope_t<R>(R((um + G)[s]), R(ux), R((upl + G)[s])) * ih;
+      g.uy[d] = slope_t<R>(R((um + 2 * G)[s]), R(uy), R((upl + 2 * G)[s])) * ih;
+      g.un[d] = slope_t<R>(R((um + 3 * G)[s]), R(un), R((upl + 3 * G)[s])) * ih;
+      g.p[d] = slope_t<R>(R((A.pin - st[d])[s]), R(p_s), R((A.pin + st[d])[s])) * ih;
     }
-    double Q[15], S[15];
+    double Qd[15];
+    R Q[15], S[15];
 #pragma unroll
-    for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? (A.qin + n * G)[s] : 0.0;
-    S[14] = 0.0;
+    for (int n = 0; n < 15; ++n) {
+      Qd[n] = (n < NV) ? q_at(A.qin, n) : 0.0;
+      Q[n] = R(Qd[n]);
+    }
+    S[14] = R(0);
     const double *up = A.uprev;
-    source_terms<EOS, BULK>(A, Q, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s],
-                            (up + 2 * G)[s], (up + 3 * G)[s], g, S);
+    source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un), R(up[s]),
+                               R((up + G)[s]), R((up + 2 * G)[s]), R((up + 3 * G)[s]), g, S);
 #pragma unroll
-    for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Q[n] + A.dt * S[n];
+    for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Qd[n] + A.dt * (double)S[n];
   } else {
 #pragma unroll
     for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (A.qout + n * G)[s];
   }
 
   // flux velocity of the cell itself for the directional sources (v = u/u^tau)
-  direction_update<EOS, BULK, 0, KEEP>(A, s, e_s, ux / ut, acc);
-  direction_update<EOS, BULK, 1, KEEP>(A, s, e_s, uy / ut, acc);
-  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP>(A, s, e_s, un / ut, acc);
+  direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, R(ux / ut), acc);
+  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS>(A, s, e_s, R(uy / ut), acc);
+  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, R(un / ut), acc);
 
   if constexpr (STAGE2) {
 #pragma unroll 1
-    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + (A.qn + n * G)[s]) * 0.5;
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + q_at(A.qn, n)) * 0.5;
   }
 
   // inferred variables with this cell's e as the guess
@@ -580,12 +608,15 @@ __global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
   if (k == NG) { gh[ng++] = -sz; gh[ng++] = -2 * sz; }
   if (k == A.nz + 1) { gh[ng++] = sz; gh[ng++] = 2 * sz; }
 
-  auto put = [&](double *base, double val) {
+  auto put = [&](auto *base, auto val) {
     base[s] = val;
     for (int m = 0; m < ng; ++m) base[(long long)s + gh[m]] = val;
   };
+  PS *qoutP = reinterpret_cast<PS *>(A.qout + 4 * G);
 #pragma unroll 1
-  for (int n = 0; n < NV; ++n) put(A.qout + n * G, acc[n * BLOCK]);
+  for (int n = 0; n < 4; ++n) put(A.qout + n * G, acc[n * BLOCK]);
+#pragma unroll 1
+  for (int n = 4; n < NV; ++n) put(qoutP + (n - 4) * G, (PS)acc[n * BLOCK]);
   put(A.eout, pr.e);
   put(A.pout, pr.p);
   put(A.uout, pr.ut);
@@ -597,47 +628,81 @@ __global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
 // ------------------------------------------------------------ launch
 // Kernel arrangement of a stage (StageArgs::variant): 0 fused kernel with
 // the inversion states kept in registers, 1 fused with states rebuilt one at
-// a time, 2 separate source kernel + flux kernel (states rebuilt).
-template <int EOS, bool BULK, bool STAGE2, bool HAS_Z>
+// a time, 2 separate source kernel + flux kernel (states rebuilt), 3 as 0
+// with the four root-finds of a direction in lockstep.  The mixed form
+// (FP32 stencil, limiter, flux and sources, FP32 pi/Pi storage, FP64
+// T^{tau mu}, accumulator, root-finds and EoS) has arrangements 0 and 3;
+// others run as 0.
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool MIXED>
 static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
   const long long ncell = (long long)A.nx * A.strip * A.nstrips * A.nz;
   const int block = BLOCK;
   const int grid = (int)(((ncell + block - 1) / block + 7) / 8 * 8);  // multiple of 8 (XCD order)
-  switch (A.variant) {
-    case 0:
-      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true>), dim3(grid),
-                         dim3(block), 0, st, A);
-      break;
-    case 1:
-      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, false>), dim3(grid),
-                         dim3(block), 0, st, A);
-      break;
-    default:
-      hipLaunchKernelGGL((kt_source_kernel<EOS, BULK, HAS_Z>), dim3(grid), dim3(block), 0, st, A);
-      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, false, false>), dim3(grid),
-                         dim3(block), 0, st, A);
+  if constexpr (MIXED) {
+    // FP32 stencil, limiter, flux and sources; FP32 pi/Pi storage
+    if (A.variant == 3)
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, true, float, float>),
+                         dim3(grid), dim3(block), 0, st, A);
+    else
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, false, float, float>),
+                         dim3(grid), dim3(block), 0, st, A);
+    return hipGetLastError();
+  } else {
+    switch (A.variant) {
+      case 0:
+        hipLaunchKernelGGL(
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, false, double, double>),
+            dim3(grid), dim3(block), 0, st, A);
+        break;
+      case 1:
+        hipLaunchKernelGGL(
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, false, false, double, double>),
+            dim3(grid), dim3(block), 0, st, A);
+        break;
+      case 3:
+        hipLaunchKernelGGL(
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, true, double, double>),
+            dim3(grid), dim3(block), 0, st, A);
+        break;
+      default:
+        hipLaunchKernelGGL((kt_source_kernel<EOS, BULK, HAS_Z>), dim3(grid), dim3(block), 0, st,
+                           A);
+        hipLaunchKernelGGL(
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, false, false, false, double, double>),
+            dim3(grid), dim3(block), 0, st, A);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
 }
 
-template <int EOS, bool BULK, bool STAGE2>
+template <int EOS, bool BULK, bool STAGE2, bool MIXED>
 static hipError_t launch_t(const StageArgs &A, bool has_z, hipStream_t st) {
-  return has_z ? launch_z<EOS, BULK, STAGE2, true>(A, st) : launch_z<EOS, BULK, STAGE2, false>(A, st);
+  return has_z ? launch_z<EOS, BULK, STAGE2, true, MIXED>(A, st)
+               : launch_z<EOS, BULK, STAGE2, false, MIXED>(A, st);
 }
 
-template <int EOS>
+template <int EOS, bool MIXED>
 static hipError_t launch_e(const StageArgs &A, bool bulk, bool stage2, bool has_z, hipStream_t st) {
-  if (bulk) return stage2 ? launch_t<EOS, true, true>(A, has_z, st) : launch_t<EOS, true, false>(A, has_z, st);
-  return stage2 ? launch_t<EOS, false, true>(A, has_z, st) : launch_t<EOS, false, false>(A, has_z, st);
+  if (bulk)
+    return stage2 ? launch_t<EOS, true, true, MIXED>(A, has_z, st)
+                  : launch_t<EOS, true, false, MIXED>(A, has_z, st);
+  return stage2 ? launch_t<EOS, false, true, MIXED>(A, has_z, st)
+                : launch_t<EOS, false, false, MIXED>(A, has_z, st);
 }
 
-hipError_t launch_stage_fp64(const StageArgs &A, int eos, bool bulk, bool stage2, hipStream_t st) {
+template <bool MIXED>
+static hipError_t launch_p(const StageArgs &A, int eos, bool bulk, bool stage2, hipStream_t st) {
   const bool has_z = A.nz > 1;
   switch (eos) {
-    case EOS_CONFORMAL: return launch_e<EOS_CONFORMAL>(A, bulk, stage2, has_z, st);
-    case EOS_WB: return launch_e<EOS_WB>(A, bulk, stage2, has_z, st);
-    default: return launch_e<EOS_TABLE>(A, bulk, stage2, has_z, st);
+    case EOS_CONFORMAL: return launch_e<EOS_CONFORMAL, MIXED>(A, bulk, stage2, has_z, st);
+    case EOS_WB: return launch_e<EOS_WB, MIXED>(A, bulk, stage2, has_z, st);
+    default: return launch_e<EOS_TABLE, MIXED>(A, bulk, stage2, has_z, st);
   }
+}
+
+hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, bool mixed,
+                        hipStream_t st) {
+  return mixed ? launch_p<true>(A, eos, bulk, stage2, st) : launch_p<false>(A, eos, bulk, stage2, st);
 }
 
 // Termination scan of the run loop (HydroPlugin.cpp:396-407): count interior
