@@ -8,8 +8,9 @@ in eta, plain FP64.  A step is one full KT step (both RK stages) over the
 whole lattice with the state resident in HBM.
 
 Prints one JSON line: cell-updates/s (interior cells x steps / wall time),
-the roofline of the stage kernel (HBM-bound stencil) and the CPU baseline
-(the FP64 oracle on the host cores, a bounded sample of the same event).
+the roofline of the stage kernel (HBM-bound stencil), the CPU baseline (the
+FP64 oracle on the host cores, a bounded sample of the same event) and the
+mixed-precision form on the same event (speed and error against FP64).
 """
 import argparse
 import json
@@ -23,30 +24,129 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-vh_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-VALUES_PER_CELL_UPDATE = {True: 107, False: 100}  # 15 (bulk) or 14 conserved variables
+
+
+def log(*a):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def algorithmic_bytes_per_cell_update(bulk, precision):
-    """Values moved per cell per KT step with perfect on-chip neighbour reuse:
+    """Bytes moved per cell per KT step with perfect on-chip neighbour reuse:
     stage 1 reads q^n, e, p, u^n, u^{n-1} and writes q*, e*, p*, u*; stage 2
-    reads q*, e*, p*, u*, u^n, q^n and writes q^{n+1}, e, p, u^{n+1}."""
-    nq = 15 if bulk else 14
-    vals = (nq + 2 + 4 + 4) + (nq + 2 + 4) + (nq + 2 + 4 + 4 + nq) + (nq + 2 + 4)
-    assert vals == VALUES_PER_CELL_UPDATE[bulk]
-    return 8 * vals
+    reads q*, e*, p*, u*, u^n, q^n and writes q^{n+1}, e, p, u^{n+1}: five
+    conserved-buffer accesses, four (e, p) and six velocity accesses.  A
+    conserved buffer is 4 FP64 T^{tau mu} plus 10 (11 with bulk) pi/Pi values
+    in FP64, or in FP32 in the mixed form."""
+    npi = 11 if bulk else 10
+    q_bytes = 4 * 8 + npi * (4 if precision == "mixed" else 8)
+    return 5 * q_bytes + 4 * 2 * 8 + 6 * 4 * 8
 
 
 def build_event(args):
     from gpuvh import ic
 
-    e = ic.c2_event(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, seed=args.seed)
-    return e
+    return ic.c2_event(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, seed=args.seed)
+
+
+def make_context(args, precision, eos, stream):
+    import gpuvh
+
+    L = gpuvh.lattice(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, args.dt)
+    H = gpuvh.hydro(args.t0, 0.16, T_fo_GeV=0.15)
+    opts = gpuvh.options(eos, True, precision, 0.16 / gpuvh.HBARC, stream=stream)
+    return gpuvh.Context(L, H, opts)
+
+
+def init_state(ctx, args, e):
+    from gpuvh import ic
+
+    ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+    p = ctx.equilibrium_pressure(e)
+    ctx.set_initial_state(args.t0, e, p, ut, ux, uy, un, pi10, Pi)
+
+
+def time_steps(ctx, args):
+    """Warm-up, then args.steps timed steps (wall clock, synchronized on both
+    sides), then args.timed_steps steps with HIP events around each stage
+    kernel.  Returns (seconds, [stage-1 ms], [stage-2 ms])."""
+    import torch
+
+    n = 0
+    ctx.steps(args.t0, n, args.warmup)
+    n += args.warmup
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ctx.steps(args.t0, n, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    n += args.steps
+    s1, s2 = [], []
+    for _ in range(args.timed_steps):
+        a, b = ctx.step_timed(args.t0 + n * args.dt)
+        n += 1
+        s1.append(a)
+        s2.append(b)
+    return elapsed, s1, s2
+
+
+def measured_traffic(workload_key):
+    """HBM bytes per step of the stage kernels from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs, calibrated against a copy kernel (profiles/traffic.json),
+    or None when no measurement of this workload is on record."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return rec.get(workload_key)
+
+
+def mixed_error(args, e, stream, nsteps):
+    """Max relative error of the mixed form against FP64 after nsteps on the
+    same event: per cell with a floor of 1% of the field's maximum (u^x, u^y,
+    u^eta and pi cross zero), and the Lorentz-norm error max|du|/u^tau."""
+    import gpuvh
+
+    ctxs = {}
+    for prec in (gpuvh.FP64, gpuvh.MIXED):
+        c = make_context(args, prec, gpuvh.EOS_TABLE, stream)
+        init_state(c, args, e)
+        c.steps(args.t0, 0, nsteps)
+        c.synchronize()
+        ctxs[prec] = c
+    out = {}
+    groups = {"e": [gpuvh.E], "u": [gpuvh.UT, gpuvh.UX, gpuvh.UY, gpuvh.UN],
+              "pi": [gpuvh.PITT, gpuvh.PITX, gpuvh.PITY, gpuvh.PITN, gpuvh.PIXX, gpuvh.PIXY,
+                     gpuvh.PIXN, gpuvh.PIYY, gpuvh.PIYN, gpuvh.PINN], "Pi": [gpuvh.PI]}
+    du = 0.0
+    for name, fields in groups.items():
+        worst = 0.0
+        for f in fields:
+            a, b = ctxs[gpuvh.FP64].get(f), ctxs[gpuvh.MIXED].get(f)
+            scale = np.abs(a).max()
+            if scale == 0:
+                continue
+            worst = max(worst, float((np.abs(b - a) / np.maximum(np.abs(a), 1e-2 * scale)).max()))
+            if name == "u" and f != gpuvh.UT:
+                du = max(du, float(np.abs(b - a).max()))
+        out[name] = worst
+    ut = ctxs[gpuvh.FP64].get(gpuvh.UT)
+    out["u_lorentz"] = du / float(np.abs(ut).min())
+    out["diagnostics_fp64"] = ctxs[gpuvh.FP64].diagnostics()
+    out["diagnostics_mixed"] = ctxs[gpuvh.MIXED].diagnostics()
+    for c in ctxs.values():
+        c.close()
+    return out
 
 
 def cpu_baseline(args, e_full, threads):
     """Time the oracle (FP64 CPU restatement of the reference KT step, test
-    infrastructure) on the host cores: one KT step of the same event on the
-    full lattice.  Cell-updates/s."""
+    infrastructure) on the host cores: KT steps of the same event on the full
+    lattice.  Cell-updates/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import gpuvh
@@ -80,14 +180,18 @@ def main():
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--t0", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="form of the headline line (the other is reported beside it)")
     ap.add_argument("--eos", choices=["table", "wb", "conformal"], default="table",
                     help="equation of state (the workload's is the 2000-node table)")
     ap.add_argument("--variant", type=int, default=None,
                     help="stage-kernel arrangement (library default if unset)")
     ap.add_argument("--timed-steps", type=int, default=5,
                     help="extra steps with HIP events around each stage kernel")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--error-steps", type=int, default=100,
+                    help="steps after which mixed is compared with FP64 (0: skip)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision report")
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -101,6 +205,7 @@ def main():
 
     if args.variant is not None:
         os.environ["GVH_VARIANT"] = str(args.variant)
+    log("start (%dx%dx%d, %s, %s)" % (args.nx, args.ny, args.nz, args.eos, args.precision))
     import torch
     import gpuvh
 
@@ -116,52 +221,66 @@ def main():
         del src, dst
 
     e = build_event(args)
-    prec = gpuvh.FP64 if args.precision == "fp64" else gpuvh.MIXED
-    L = gpuvh.lattice(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, args.dt)
-    H = gpuvh.hydro(args.t0, 0.16, T_fo_GeV=0.15)
+    log("event built")
     eos = {"table": gpuvh.EOS_TABLE, "wb": gpuvh.EOS_WB, "conformal": gpuvh.EOS_CONFORMAL}[args.eos]
-    opts = gpuvh.options(eos, True, prec, 0.16 / gpuvh.HBARC, stream=stream.cuda_stream)
-    ctx = gpuvh.Context(L, H, opts)
-    ut, ux, uy, un, pi10, Pi = gpuvh.ic.at_rest(e.shape)
-    p = ctx.equilibrium_pressure(e)
-    ctx.set_initial_state(args.t0, e, p, ut, ux, uy, un, pi10, Pi)
-
-    n = 0
-    ctx.steps(args.t0, n, args.warmup)
-    n += args.warmup
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    ctx.steps(args.t0, n, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    n += args.steps
-
-    # per-launch durations of the two stage kernels, HIP events on the stream
-    # the kernels run on
-    s1, s2 = [], []
-    for _ in range(args.timed_steps):
-        a, b = ctx.step_timed(args.t0 + n * args.dt)
-        n += 1
-        s1.append(a)
-        s2.append(b)
-    diag = ctx.diagnostics()
     ncells = args.nx * args.ny * args.nz
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = ncells * args.steps / elapsed
+    forms = [args.precision] + ([] if args.no_mixed else
+                                [p for p in ("fp64", "mixed") if p != args.precision])
+    res = {}
+    diag = None
+    for form in forms:
+        prec = gpuvh.FP64 if form == "fp64" else gpuvh.MIXED
+        ctx = make_context(args, prec, eos, stream.cuda_stream)
+        init_state(ctx, args, e)
+        elapsed, s1, s2 = time_steps(ctx, args)
+        if form == args.precision:
+            diag = ctx.diagnostics()
+        ctx.close()
+        bpc = algorithmic_bytes_per_cell_update(True, form)
+        kt_ms = float(np.mean(s1) + np.mean(s2))
+        log("%s: %.4f ms/step, stage kernels %.4f + %.4f ms"
+            % (form, 1e3 * elapsed / args.steps, np.mean(s1), np.mean(s2)))
+        res[form] = {"ms_per_step": 1e3 * elapsed / args.steps,
+                     "value": ncells * args.steps / elapsed,
+                     "stage_ms": [float(np.mean(s1)), float(np.mean(s2))],
+                     "kernel_ms": kt_ms, "bytes_per_cell_update": bpc,
+                     "achieved_gbs": bpc * ncells / (kt_ms * 1e-3) / 1e9}
 
-    bpc = algorithmic_bytes_per_cell_update(True, prec)
-    kt_ms = float(np.mean(s1) + np.mean(s2))
-    achieved = bpc * ncells / (kt_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "kt_stage_kernel", "stage_ms": [round(float(np.mean(s1)), 4),
-                                                          round(float(np.mean(s2)), 4)],
-                "bytes_per_cell_update": bpc}
+    head = res[args.precision]
+    wkey = "%dx%dx%d/%s/%s/v%s" % (args.nx, args.ny, args.nz, args.eos, args.precision,
+                                   os.environ.get("GVH_VARIANT", "default"))
+    traffic = measured_traffic(wkey)
+    roofline = {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": "kt_flux_kernel (one launch per RK stage)",
+                "stage_ms": [round(v, 4) for v in head["stage_ms"]],
+                "bytes_per_cell_update": head["bytes_per_cell_update"],
+                "cells_per_launch": ncells}
+
+    mixed = None
+    if not args.no_mixed:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        m = res["mixed"]
+        mixed = {"ms_per_step": round(m["ms_per_step"], 4), "value": round(m["value"], 1),
+                 "speedup_vs_fp64": round(res["fp64"]["ms_per_step"] / m["ms_per_step"], 3),
+                 "roofline_frac": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                 "bytes_per_cell_update": m["bytes_per_cell_update"], "reported_beside": other}
+        if args.error_steps > 0 and args.eos == "table":
+            log("mixed vs fp64 after %d steps" % args.error_steps)
+            errs = mixed_error(args, e, stream.cuda_stream, args.error_steps)
+            mixed["error_vs_fp64"] = {k: (float("%.3e" % v) if isinstance(v, float) else v)
+                                      for k, v in errs.items()}
+            mixed["error_steps"] = args.error_steps
+            mixed["error_metric"] = ("max per-cell |mixed - fp64| / max(|fp64|, 1e-2 max|fp64|) "
+                                     "per field group; u_lorentz = max|du^i| / min u^tau")
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d oracle steps on %d threads" % (args.cpu_steps, threads))
         v, el, ns = cpu_baseline(args, e, threads)
+        log("cpu baseline: %.3g cell-updates/s" % v)
         cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port",
                "sample": "oracle (FP64 CPU restatement of the reference KT step, OpenMP) on "
@@ -170,29 +289,29 @@ def main():
 
     out = {
         "metric": "cell-updates/s per KT step",
-        "value": round(value, 1),
+        "value": round(head["value"], 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == gpuvh.FP64 else "f32/f64 mixed",
+        "dtype": "f64" if args.precision == "fp64" else "f32/f64 mixed",
         "data": "synthetic",
         "config": {"workload": "3+1D KT step %dx%dx%d, %s EoS, shear+bulk, %s"
                                % (args.nx, args.ny, args.nz,
                                   "tabulated" if args.eos == "table" else args.eos, args.precision),
                    "lattice": [args.nx, args.ny, args.nz], "dtau": args.dt, "tau0": args.t0,
                    "eos": "tabulated 2000 nodes" if args.eos == "table" else args.eos,
-                   "eta_over_s": 0.16,
-                   "zeta_peak_GeV": 0.16, "seed": args.seed, "parallelism": "single GPU"},
+                   "eta_over_s": 0.16, "zeta_peak_GeV": 0.16, "seed": args.seed,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "diagnostics": diag,
     }
-    ctx.close()
     print(json.dumps(out))
 
 

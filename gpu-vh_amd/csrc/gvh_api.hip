@@ -497,20 +497,25 @@ extern "C" int gvh_get_field(gvh_context *c, int field, int ghosted, double *hos
 }
 
 // Arguments of one stage launch for the current buffer roles.
-static StageArgs stage_args(gvh_context *c, int stage, double t) {
+namespace gvh {
+Buffers context_buffers(const gvh_context *c) {
+  Buffers B;
+  for (int b = 0; b < 3; ++b) {
+    B.q[b] = c->qbuf[b];
+    B.u[b] = c->ubuf[b];
+  }
+  for (int b = 0; b < 2; ++b) {
+    B.e[b] = c->ebuf[b];
+    B.p[b] = c->pbuf[b];
+  }
+  return B;
+}
+
+Roles context_roles(const gvh_context *c) { return Roles{c->iq, c->iu, c->iup, c->ie}; }
+
+StageArgs make_stage_args(gvh_context *c, int stage, double t) {
   StageArgs A{};
-  const int ie_in = stage == 1 ? c->ie : 1 - c->ie;
-  const int q_in = stage == 1 ? c->iq : c->iqs();
-  A.qin = c->qbuf[q_in];
-  A.qn = c->qbuf[c->iq];
-  A.qout = c->qbuf[stage == 1 ? c->iqs() : c->iQ()];
-  A.uin = c->ubuf[stage == 1 ? c->iu : c->ius()];
-  A.uprev = c->ubuf[stage == 1 ? c->iup : c->iu];
-  A.uout = c->ubuf[stage == 1 ? c->ius() : c->iup];
-  A.ein = c->ebuf[ie_in];
-  A.pin = c->pbuf[ie_in];
-  A.eout = c->ebuf[1 - ie_in];
-  A.pout = c->pbuf[1 - ie_in];
+  stage_pointers(A, context_buffers(c), context_roles(c), stage);
   A.diag = c->diag;
   A.G = c->G;
   A.nx = c->nx;
@@ -519,10 +524,9 @@ static StageArgs stage_args(gvh_context *c, int stage, double t) {
   A.ncx = c->ncx;
   A.ncy = c->ncy;
   const double dt = c->L.latticeSpacingProperTime;
-  A.t = stage == 1 ? t : t + dt;
-  A.tinf = t + dt;
   A.dt = dt;
   A.inv_dt = 1.0 / dt;
+  stage_times(A, t, stage);
   const double h[3] = {c->L.latticeSpacingX, c->L.latticeSpacingY, c->L.latticeSpacingRapidity};
   for (int d = 0; d < 3; ++d) {
     A.h[d] = h[d];
@@ -538,6 +542,7 @@ static StageArgs stage_args(gvh_context *c, int stage, double t) {
   A.variant = c->variant;
   return A;
 }
+}  // namespace gvh
 
 int launch_stage(gvh_context *c, const StageArgs &A, bool stage2);
 
@@ -545,10 +550,11 @@ int launch_stage(gvh_context *c, const StageArgs &A, bool stage2);
 // writes Q, e/p[ie], and u^{n+1} into the u^{n-1} buffer; then the roles
 // swap like setCurrentConservedVariables and swapFluidVelocity.
 static void rotate_after_step(gvh_context *c) {
-  const int old_u = c->iu;
-  c->iu = c->iup;   // u^{n+1} was written into the u^{n-1} buffer
-  c->iup = old_u;   // u^n becomes the previous-step velocity
-  c->iq = c->iQ();  // q^{n+1} is in Q; the old q^n buffer becomes Q
+  Roles R = context_roles(c);
+  rotate_roles(R);
+  c->iq = R.iq;
+  c->iu = R.iu;
+  c->iup = R.iup;
   ++c->steps_done;
 }
 
@@ -557,11 +563,11 @@ static int step_impl(gvh_context *c, double t, float *ms1, float *ms2) {
   if (!c->initialized) return gvh_fail(GVH_EINVAL, "state not initialised (gvh_set_initial_state)");
   const bool timed = ms1 || ms2;
   if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  StageArgs A1 = stage_args(c, 1, t);
+  StageArgs A1 = make_stage_args(c, 1, t);
   int rc = launch_stage(c, A1, false);
   if (rc) return rc;
   if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  StageArgs A2 = stage_args(c, 2, t);
+  StageArgs A2 = make_stage_args(c, 2, t);
   rc = launch_stage(c, A2, true);
   if (rc) return rc;
   if (timed) {

@@ -60,6 +60,73 @@ struct StageArgs {
   int variant;          // kernel arrangement of a stage (gvh_stage.hip)
 };
 
+// Buffers of one event and their current roles: the reference's pointer
+// swaps (FullyDiscreteKurganovTadmorScheme.cu:117-133,
+// DynamicalVariables.cu:400-408) as indices.  q[iq] holds q^n, q[1-iq] the
+// corrector output Q, q[2] the predictor qS; u[iu] u^n, u[iup] u^{n-1},
+// u[3-iu-iup] the predictor u*; (e, p)[ie] the current state.
+struct Roles {
+  int iq, iu, iup, ie;
+};
+struct Buffers {
+  double *q[3], *u[3], *e[2], *p[2];
+};
+
+// State pointers of RK stage 1 or 2 for the given roles.
+__host__ __device__ inline void stage_pointers(StageArgs &A, const Buffers &B, const Roles &R,
+                                               int stage) {
+  const int iqs = 2, iQ = 1 - R.iq, ius = 3 - R.iu - R.iup;
+  const int ie_in = stage == 1 ? R.ie : 1 - R.ie;
+  A.qin = B.q[stage == 1 ? R.iq : iqs];
+  A.qn = B.q[R.iq];
+  A.qout = B.q[stage == 1 ? iqs : iQ];
+  A.uin = B.u[stage == 1 ? R.iu : ius];
+  A.uprev = B.u[stage == 1 ? R.iup : R.iu];
+  A.uout = B.u[stage == 1 ? ius : R.iup];
+  A.ein = B.e[ie_in];
+  A.pin = B.p[ie_in];
+  A.eout = B.e[1 - ie_in];
+  A.pout = B.p[1 - ie_in];
+}
+
+// Times of stage 1 or 2 of the step that starts at t (the predictor's
+// inferred variables are solved at t + dt, FullyDiscreteKurganovTadmorScheme.cu:119).
+__host__ __device__ inline void stage_times(StageArgs &A, double t, int stage) {
+  A.t = stage == 1 ? t : t + A.dt;
+  A.tinf = t + A.dt;
+}
+
+// Roles after a step: u^{n+1} was written into the u^{n-1} buffer, q^{n+1}
+// into Q.
+__host__ __device__ inline void rotate_roles(Roles &R) {
+  const int old_u = R.iu;
+  R.iu = R.iup;
+  R.iup = old_u;
+  R.iq = 1 - R.iq;
+}
+
+// One event of a batch in device memory.
+enum { SLOT_FREE = 0, SLOT_RUN = 1, SLOT_DONE = 2 };
+struct BatchSlot {
+  StageArgs A;        // constants of the event; pointers and times are set per stage
+  Buffers B;
+  Roles R;
+  double t0;
+  int steps;          // KT steps done
+  int status;         // SLOT_*
+  int max_steps;      // the run loop's nt + 1
+  int frozen;         // steps seen with no interior cell above e_fo (never reset)
+  int fofreq;
+  int fo_flags;       // GVH_FO_CURRENT_FIELDS selects the current u, pi for the ring
+  int frozen_stop;    // 1 if it ended by the all-frozen rule
+  int bulk;           // Pi is evolved
+  int tile_first, ntiles;  // its tiles (blocks of cells) in the tile map
+  double e_fo;
+  unsigned long long above;  // interior cells with e > e_fo in the current state
+  double *ring;       // freeze-out history, (fofreq+1) x 10 x nint, or nullptr
+  long long nint;
+};
+
 // ------------------------------------------------------------ limiter
 // approximateDerivative (FluxLimiter.cu:33-39): the nested generalized
 // minmod(theta*dl, minmod(dc, theta*dr)) with sgn(0)=+1 equals
@@ -276,11 +343,19 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
   if constexpr (EOS == EOS_CONFORMAL) {
     return fabs(sqrt(fabs(4.0 * M0 * M0 - 3.0 * M)) - M0);
   } else {
+    // A NaN iterate never meets the stopping rule and every later iterate is
+    // NaN, so the reference's 10^6 iterations end on NaN: leaving at the
+    // first NaN returns the same value.
     double e = ePrev;
-    for (int it = 0; it < MAX_ITERS / 2; ++it)
+    bool nan = false;
+    for (int it = 0; it < MAX_ITERS / 2 && !nan; ++it) {
       if (newton_step<EOS, false>(T, e, M0, M, Pi)) return e;
-    for (int it = MAX_ITERS / 2; it < MAX_ITERS; ++it)
+      nan = isnan(e);
+    }
+    for (int it = MAX_ITERS / 2; it < MAX_ITERS && !nan; ++it) {
       if (newton_step<EOS, true>(T, e, M0, M, Pi)) return e;
+      nan = isnan(e);
+    }
     atomicAdd(&diag[DIAG_NOT_CONVERGED], 1ull);
     return e;  // the last iterate, as the reference returns e0 after e0 = e
   }
@@ -303,6 +378,7 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev
     unsigned act = (1u << N) - 1u;
 #pragma unroll
     for (int k = 0; k < N; ++k) e[k] = ePrev;
+    unsigned failed = 0;  // chains that reached a NaN iterate (see solve_e)
     for (int it = 0; it < MAX_ITERS / 2 && act; ++it) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
@@ -311,6 +387,10 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
+        if (live && !done && isnan(ek)) {
+          act &= ~(1u << k);
+          failed |= 1u << k;
+        }
       }
     }
     for (int it = MAX_ITERS / 2; it < MAX_ITERS && act; ++it) {
@@ -321,8 +401,13 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
+        if (live && !done && isnan(ek)) {
+          act &= ~(1u << k);
+          failed |= 1u << k;
+        }
       }
     }
+    act |= failed;
     if (act) atomicAdd(&diag[DIAG_NOT_CONVERGED], (unsigned long long)__popc(act));
   }
 }

@@ -83,6 +83,8 @@ struct FoState {
   long long ncubes = 0, nrows = 0;
 };
 
+double *fo_ring(FoState *f) { return f ? f->ring : nullptr; }
+
 void fo_free(FoState *f) {
   if (!f) return;
   for (void *p : {(void *)f->ring, (void *)f->rowcnt, (void *)f->rowoff, (void *)f->cubes,

@@ -35,6 +35,14 @@ struct gvh_context;
 // the full ghosted grid, whatever its storage precision.
 int gvh_copy_q_field(gvh_context *c, const double *qbase, int f, double *host_ghosted);
 
+namespace gvh {
+// Stage launch arguments of a context for stage 1 or 2 of the step at t.
+StageArgs make_stage_args(gvh_context *c, int stage, double t);
+Buffers context_buffers(const gvh_context *c);
+Roles context_roles(const gvh_context *c);
+double *fo_ring(FoState *f);
+}  // namespace gvh
+
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t _e = (expr);                                                        \

@@ -1,0 +1,643 @@
+// gvh_stage_body.h -- one fused kernel per Runge-Kutta stage of the KT step
+// (launched by gvh_stage.hip for one event and by gvh_batch_kernels.hip for
+// batches of events).
+//
+// The reference runs, per stage, a Source kernel, X/Y/Z flux kernels that
+// read-modify-write the output, an inferred-variables kernel, a regulation
+// kernel, three ghost-cell kernels on transient streams and (stage 2) a
+// convex-average kernel (FullyDiscreteKurganovTadmorScheme.cu:109-139).
+// Here one thread owns one interior cell and does all of it in registers:
+//
+//   acc = q + dt*S                    (eulerStepKernelSource_1D, EulerStep.cu:947)
+//   acc += dt*(flux diff + S_x)       (eulerStepKernelX_1D, EulerStep.cu:1012)
+//   acc += dt*(flux diff + S_y)       (..Y_1D, :1101)
+//   acc += dt*(flux diff + S_z)       (..Z_1D, :1190)
+//   acc = (acc + q^n)/2               (stage 2, convexCombination..., :60-104)
+//   (e,p,u) = root-find(acc; e_in)    (setInferredVariablesKernel)
+//   pi *= regulation factor           (regulateDissipativeCurrents)
+//   write acc, e, p, u; boundary cells also write their two ghost layers
+//
+// The summation order of the reference is kept.  Inputs and outputs are
+// distinct buffers (q, e, p and u are ping-ponged by the host), so no thread
+// reads what another writes within a launch.
+#pragma once
+#include "../../include/gpuvh.h"
+#include "gvh_device.h"
+
+namespace gvh {
+
+// ------------------------------------------------------ source terms
+template <typename R>
+struct GradT {
+  R ut[3], ux[3], uy[3], un[3], p[3];  // [x, y, eta] derivatives
+};
+using Grad = GradT<double>;
+
+// loadSourceTerms2 (SourceTerms.cu:570-755) with setPimunuSourceTerms
+// (:30-256): T^{tau mu} geometric and pressure-gradient sources and the
+// relaxation right-hand sides of pi^{mu nu} and Pi.
+template <int EOS, bool BULK, typename R>
+__device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, double e_eos,
+                                             R e, R p, R ut, R ux, R uy, R un, R utp, R uxp,
+                                             R uyp, R unp, const GradT<R> &g, R *S) {
+  const R t = R(A.t);
+  const R inv_t = R(1.0) / t;
+  const R t2 = t * t, t3 = t * t2;
+  const R inv_t2 = R(1.0) / t2;
+  const R Pi = BULK ? Q[14] : R(0.0);
+  const R pitt = Q[4], pitx = Q[5], pity = Q[6], pitn = Q[7], pixx = Q[8],
+               pixy = Q[9], pixn = Q[10], piyy = Q[11], piyn = Q[12], pinn = Q[13];
+  const R inv_ut = R(1.0) / ut;
+
+  // T^{tau mu} sources, SourceTerms.cu:726-737
+  const R PP = p + Pi;
+  const R tnn = (e + PP) * un * un + PP * inv_t2 + pinn;
+  const R vx = ux * inv_ut, vy = uy * inv_ut, vn = un * inv_ut;
+  const R dkvk = (g.ux[0] - vx * g.ut[0]) * inv_ut + (g.uy[1] - vy * g.ut[1]) * inv_ut +
+                      (g.un[2] - vn * g.ut[2]) * inv_ut;
+  S[0] = -(Q[0] * inv_t + t * tnn) + dkvk * (pitt - p - Pi) - vx * g.p[0] - vy * g.p[1] -
+         vn * g.p[2];
+  S[1] = -Q[1] * inv_t - g.p[0] + dkvk * pitx;
+  S[2] = -Q[2] * inv_t - g.p[1] + dkvk * pity;
+  S[3] = -R(3.0) * Q[3] * inv_t - g.p[2] * inv_t2 + dkvk * pitn;
+
+  // transport coefficients, SourceTerms.cu:46-60
+  const double T_eos = eos_T<EOS>(A.tab, e_eos);
+  const R T = R(T_eos);
+  const R taupiInv = R(0.2f) * (T / R(A.etabar));
+  const R beta_pi = R(0.2f) * (e + p);
+
+  // backward-difference time derivative and covariant acceleration, :67-86
+  const R dtut = (ut - utp) * R(A.inv_dt), dtux = (ux - uxp) * R(A.inv_dt);
+  const R dtuy = (uy - uyp) * R(A.inv_dt), dtun = (un - unp) * R(A.inv_dt);
+  const R un2 = un * un, ut2 = ut * ut;
+  const R dut = ut * dtut + ux * g.ut[0] + uy * g.ut[1] + un * g.ut[2];
+  const R Dut = dut + t * un2;
+  const R dux = ut * dtux + ux * g.ux[0] + uy * g.ux[1] + un * g.ux[2];
+  const R duy = ut * dtuy + ux * g.uy[0] + uy * g.uy[1] + un * g.uy[2];
+  const R dun = ut * dtun + ux * g.un[0] + uy * g.un[1] + un * g.un[2];
+  const R Dux = -dux, Duy = -duy;
+  const R Dun = -t2 * dun - R(2.0) * t * ut * un;
+
+  const R theta = ut * inv_t + dtut + g.ux[0] + g.uy[1] + g.un[2];
+  const R th3 = theta * (R(1.0) / R(3.0));
+
+  // sigma^{mu nu}, :96-106
+  const R stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - R(1.0)) * th3;
+  const R stx = -(t * un2 * ux) * R(0.5) + (dtux - g.ut[0]) * R(0.5) -
+                     (ux * dut + ut * dux) * R(0.5) + ut * ux * th3;
+  const R sty = -(t * un2 * uy) * R(0.5) + (dtuy - g.ut[1]) * R(0.5) -
+                     (uy * dut + ut * duy) * R(0.5) + ut * uy * th3;
+  const R stn = -un * (R(2.0) * ut2 + t2 * un2) * (R(0.5) * inv_t) +
+                     (dtun - g.ut[2] * inv_t2) * R(0.5) - (un * dut + ut * dun) * R(0.5) +
+                     ut * un * th3;
+  const R sxx = -(g.ux[0] + ux * dux) + (R(1.0) + ux * ux) * th3;
+  const R sxy = -(g.uy[0] + g.ux[1]) * R(0.5) - (uy * dux + ux * duy) * R(0.5) + ux * uy * th3;
+  const R sxn = -ut * ux * un * inv_t - (g.un[0] + g.ux[2] * inv_t2) * R(0.5) -
+                     (un * dux + ux * dun) * R(0.5) + ux * un * th3;
+  const R syy = -(g.uy[1] + uy * duy) + (R(1.0) + uy * uy) * th3;
+  const R syn = -ut * uy * un * inv_t - (g.un[1] + g.uy[2] * inv_t2) * R(0.5) -
+                     (un * duy + uy * dun) * R(0.5) + uy * un * th3;
+  const R snn = -ut * (R(1.0) + R(2.0) * t2 * un2) / t3 - g.un[2] * inv_t2 - un * dun +
+                     (inv_t2 + un2) * th3;
+
+  // omega^mu_nu, :123-136
+  const R wtx = (-dtux - g.ut[0] + ut * dux - ux * dut - t * un2 * ux) * R(0.5);
+  const R wty = (-dtuy - g.ut[1] + ut * duy - uy * dut - t * un2 * uy) * R(0.5);
+  const R wtn = (-t2 * dtun - R(2.0) * t * un - g.ut[2] - ut * Dun - t2 * un * dut -
+                      t3 * un * un2) * R(0.5);
+  const R wxy = (g.uy[0] - g.ux[1] + ux * duy - uy * dux) * R(0.5);
+  const R wxn = (t2 * g.un[0] - g.ux[2] - ux * Dun - t2 * un * dux) * R(0.5);
+  const R wyn = (t2 * g.un[1] - g.uy[2] - uy * Dun - t2 * un * duy) * R(0.5);
+  const R wxt = wtx, wyt = wty, wnt = wtn * inv_t2, wyx = -wxy;
+  const R wnx = -wxn * inv_t2, wny = -wyn * inv_t2;
+
+  // pi^{mu lambda} D u_lambda for each row mu, then I1 = u^mu a^nu + u^nu a^mu
+  const R at = pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun;
+  const R ax = pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun;
+  const R ay = pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun;
+  const R an = pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun;
+
+  // pi:sigma, :185-187
+  const R ps = pitt * stt - R(2.0) * pitx * stx - R(2.0) * pity * sty + pixx * sxx +
+                    R(2.0) * pixy * sxy + piyy * syy - R(2.0) * pitn * stn * t2 +
+                    R(2.0) * pixn * sxn * t2 + R(2.0) * piyn * syn * t2 + pinn * snn * t2 * t2;
+  const R ps3 = ps * (R(1.0) / R(3.0));
+
+  constexpr R d_pp = R(1.33333), t_pp = R(1.42857), l_pP = R(1.2);  // TransportCoefficients.cuh:14-16
+  const R lPi = l_pP * Pi;
+
+  // I = I1 + delta I2 - I3 + tau I4 - lambda Pi sigma, relaxation RHS with
+  // Milne Christoffel terms, :141-233; RHS/u^tau + pi dkvk, :243-252
+  const R b2 = R(2.0) * beta_pi;
+  {
+    const R I1 = R(2.0) * ut * at;
+    const R I3 = R(2.0) * (pitx * wtx + pity * wty + pitn * wtn);
+    const R I4 = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - (R(1.0) - ut2) * ps3;
+    const R I = I1 + d_pp * theta * pitt - I3 + t_pp * I4 - lPi * stt;
+    const R r = b2 * stt - pitt * taupiInv - I - R(2.0) * un * t * pitn;
+    S[4] = r * inv_ut + pitt * dkvk;
+  }
+  {
+    const R I1 = ux * at + ut * ax;
+    const R I3 = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
+    const R I4 = (pitt * stx + pitx * stt) * R(0.5) - (pitx * sxx + pixx * stx) * R(0.5) -
+                      (pity * sxy + pixy * sty) * R(0.5) - t2 * (pitn * sxn + pixn * stn) * R(0.5) +
+                      (ut * ux) * ps3;
+    const R I = I1 + d_pp * theta * pitx - I3 + t_pp * I4 - lPi * stx;
+    const R r = b2 * stx - pitx * taupiInv - I - un * t * pixn;
+    S[5] = r * inv_ut + pitx * dkvk;
+  }
+  {
+    const R I1 = uy * at + ut * ay;
+    const R I3 = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
+    const R I4 = (pitt * sty + pity * stt) * R(0.5) - (pitx * sxy + pixy * stx) * R(0.5) -
+                      (pity * syy + piyy * sty) * R(0.5) - t2 * (pitn * syn + piyn * stn) * R(0.5) +
+                      (ut * uy) * ps3;
+    const R I = I1 + d_pp * theta * pity - I3 + t_pp * I4 - lPi * sty;
+    const R r = b2 * sty - pity * taupiInv - I - un * t * piyn;
+    S[6] = r * inv_ut + pity * dkvk;
+  }
+  {
+    const R I1 = un * at + ut * an;
+    const R I3 = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
+    const R I4 = (pitt * stn + pitn * stt) * R(0.5) - (pitx * sxn + pixn * stx) * R(0.5) -
+                      (pity * syn + piyn * sty) * R(0.5) - t2 * (pitn * snn + pinn * stn) * R(0.5) +
+                      (ut * un) * ps3;
+    const R I = I1 + d_pp * theta * pitn - I3 + t_pp * I4 - lPi * stn;
+    const R r = b2 * stn - pitn * taupiInv - I - un * t * pinn -
+                     (ut * pitn + un * pitt) * inv_t;
+    S[7] = r * inv_ut + pitn * dkvk;
+  }
+  {
+    const R I1 = R(2.0) * ux * ax;
+    const R I3 = R(2.0) * (pitx * wxt + pixy * wxy + pixn * wxn);
+    const R I4 = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + (R(1.0) + ux * ux) * ps3;
+    const R I = I1 + d_pp * theta * pixx - I3 + t_pp * I4 - lPi * sxx;
+    const R r = b2 * sxx - pixx * taupiInv - I;
+    S[8] = r * inv_ut + pixx * dkvk;
+  }
+  {
+    const R I1 = uy * ax + ux * ay;
+    const R I3 = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
+    const R I4 = (pitx * sty + pity * stx) * R(0.5) - (pixx * sxy + pixy * sxx) * R(0.5) -
+                      (pixy * syy + piyy * sxy) * R(0.5) - t2 * (pixn * syn + piyn * sxn) * R(0.5) +
+                      (ux * uy) * ps3;
+    const R I = I1 + d_pp * theta * pixy - I3 + t_pp * I4 - lPi * sxy;
+    const R r = b2 * sxy - pixy * taupiInv - I;
+    S[9] = r * inv_ut + pixy * dkvk;
+  }
+  {
+    const R I1 = un * ax + ux * an;
+    const R I3 = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
+    const R I4 = (pitx * stn + pitn * stx) * R(0.5) - (pixx * sxn + pixn * sxx) * R(0.5) -
+                      (pixy * syn + piyn * sxy) * R(0.5) - t2 * (pixn * snn + pinn * sxn) * R(0.5) +
+                      (ux * un) * ps3;
+    const R I = I1 + d_pp * theta * pixn - I3 + t_pp * I4 - lPi * sxn;
+    const R r = b2 * sxn - pixn * taupiInv - I - (ut * pixn + un * pitx) * inv_t;
+    S[10] = r * inv_ut + pixn * dkvk;
+  }
+  {
+    const R I1 = R(2.0) * uy * ay;
+    const R I3 = R(2.0) * (pity * wyt + pixy * wyx + piyn * wyn);
+    const R I4 = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + (R(1.0) + uy * uy) * ps3;
+    const R I = I1 + d_pp * theta * piyy - I3 + t_pp * I4 - lPi * syy;
+    const R r = b2 * syy - piyy * taupiInv - I;
+    S[11] = r * inv_ut + piyy * dkvk;
+  }
+  {
+    const R I1 = un * ay + uy * an;
+    const R I3 = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
+    const R I4 = (pity * stn + pitn * sty) * R(0.5) - (pixy * sxn + pixn * sxy) * R(0.5) -
+                      (piyy * syn + piyn * syy) * R(0.5) - t2 * (piyn * snn + pinn * syn) * R(0.5) +
+                      (uy * un) * ps3;
+    const R I = I1 + d_pp * theta * piyn - I3 + t_pp * I4 - lPi * syn;
+    const R r = b2 * syn - piyn * taupiInv - I - (ut * piyn + un * pity) * inv_t;
+    S[12] = r * inv_ut + piyn * dkvk;
+  }
+  {
+    const R I1 = R(2.0) * un * an;
+    const R I3 = R(2.0) * (pitn * wnt + pixn * wnx + piyn * wny);
+    const R I4 = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + (inv_t2 + un2) * ps3;
+    const R I = I1 + d_pp * theta * pinn - I3 + t_pp * I4 - lPi * snn;
+    const R r = b2 * snn - pinn * taupiInv - I - R(2.0) * (ut * pinn + un * pitn) * inv_t;
+    S[13] = r * inv_ut + pinn * dkvk;
+  }
+  if constexpr (BULK) {
+    // bulk relaxation, :53-60 and :238, :254
+    double p_, cs2;
+    eos_p_cs2<EOS>(A.tab, e_eos, p_, cs2);
+    const R a = R(0.333333f) - R(cs2);
+    const R a2 = a * a;
+    const R beta_Pi = R(15.0) * a2 * (e + p);
+    const R lambda_Pipi = R(1.6f) * a;
+    const R tauPiInv = R(15.0) * a2 * (T / R(zeta_over_s(T_eos, A.zeta_tpeak)));
+    constexpr R d_PP = R(0.666667);  // TransportCoefficients.cuh:18
+    const R dPi = -beta_Pi * theta - Pi * tauPiInv - d_PP * Pi * theta + lambda_Pipi * ps;
+    S[14] = dPi * inv_ut + Pi * dkvk;
+  }
+}
+
+// ------------------------------------------------- directional update
+// Threads per block of the stage kernel; the per-thread accumulator of the 15
+// conserved variables lives in LDS as acc[n * BLOCK + threadIdx.x] so that the
+// variable loops can stay rolled (dynamic index) and registers stay few.
+constexpr int BLOCK = 256;
+
+// One direction of eulerStepKernel{X,Y,Z}_1D: the two KT interface fluxes of
+// flux() (SemiDiscreteKurganovTadmorScheme.cu:16-57), each with both
+// half-site states inverted using this cell's e as the guess, plus the
+// central-difference pi / Pi gradient sources of loadSourceTerms{X,Y,Z}
+// (SourceTerms.cu:444-567).  acc[n] += ((H_b - H_f)/h + S_dir) dt.
+//
+// Half-site states (HalfSiteExtrapolation.cu:16-29): right/left at i+1/2
+// are q_{i+1} - D_{i+1}/2 and q_i + D_i/2, right/left at i-1/2 are
+// q_i - D_i/2 and q_{i-1} + D_{i-1}/2.  The nine variables the inversion
+// needs (T^{tau mu}, pi^{tau mu}, Pi) are reconstructed state by state for
+// the four solves, then every variable once more for its fluxes.
+// Every stencil load is a uniform (scalar) base plus the thread's 32-bit
+// cell offset.
+template <int EOS, bool BULK, int DIR, bool KEEP_STATES, bool LOCK, typename R, typename PS>
+__device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s, double ePrev,
+                                                 R v_s, double *acc) {
+  constexpr int NV = BULK ? 15 : 14;
+  const long long st = DIR == 0 ? 1 : (DIR == 1 ? (long long)A.ncx : (long long)A.ncx * A.ncy);
+  const long long G = A.G;
+  const double *__restrict__ qT = A.qin;                                    // T^{tau mu}
+  const PS *__restrict__ qP = reinterpret_cast<const PS *>(A.qin + 4 * G);  // pi, Pi
+  const double t = A.t;
+  const R hc = R(A.half_inv_h[DIR]);
+  // conserved variable n at cell s + o (n < 4: T^{tau mu}, else pi / Pi)
+  auto ld = [&](int n, long long o) -> R {
+    return n < 4 ? R((qT + n * G + o)[s]) : R((qP + (n - 4) * G + o)[s]);
+  };
+  auto ldp = [&](int n, long long o) -> R { return R((qP + (n - 4) * G + o)[s]); };
+
+  double vRf, vLf, vRb, vLb;
+  // the four states when KEEP_STATES: [0] right of i+1/2, [1] left of i+1/2,
+  // [2] right of i-1/2, [3] left of i-1/2
+  R Wst[4][9];
+  R cdv[5];
+  if constexpr (KEEP_STATES) {
+    // all four states of the nine inversion variables stay in registers
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      if (a == 8 && !BULK) {
+        Wst[0][8] = Wst[1][8] = Wst[2][8] = Wst[3][8] = cdv[4] = R(0);
+        continue;
+      }
+      const int n = a < 8 ? a : 14;
+      const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
+              vp2 = ld(n, 2 * st);
+      const R D0 = slope_t<R>(vm1, v0, vp1);
+      Wst[0][a] = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
+      Wst[1][a] = v0 + D0 * R(0.5);
+      Wst[2][a] = v0 - D0 * R(0.5);
+      Wst[3][a] = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
+      if (a >= 4) cdv[a - 4] = (vp1 - vm1) * hc;
+    }
+    double v4[4];
+    if constexpr (LOCK) {
+      flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, v4);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v4[k] = flux_velocity<EOS, DIR>(A.tab, t, (double)Wst[k][0], (double)Wst[k][1],
+                                        (double)Wst[k][2], (double)Wst[k][3], (double)Wst[k][4],
+                                        (double)Wst[k][5], (double)Wst[k][6], (double)Wst[k][7],
+                                        (double)Wst[k][8], ePrev, A.diag);
+    }
+    vRf = v4[0];
+    vLf = v4[1];
+    vRb = v4[2];
+    vLb = v4[3];
+  } else {
+    // the four states one after another (rolled: few live registers)
+    vRf = vLf = vRb = vLb = 0.0;
+#pragma unroll 1
+    for (int state = 0; state < 4; ++state) {
+      const long long o = state == 0 ? st : (state == 3 ? -st : 0);
+      const R sg = (state & 1) ? R(0.5) : R(-0.5);
+      R w[9];
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        if (a == 8 && !BULK) {
+          w[8] = R(0);
+          continue;
+        }
+        const int n = a < 8 ? a : 14;
+        const R xm = ld(n, o - st), x0 = ld(n, o), xp = ld(n, o + st);
+        w[a] = x0 + slope_t<R>(xm, x0, xp) * sg;
+      }
+      const double v = flux_velocity<EOS, DIR>(A.tab, t, (double)w[0], (double)w[1], (double)w[2],
+                                               (double)w[3], (double)w[4], (double)w[5],
+                                               (double)w[6], (double)w[7], (double)w[8], ePrev,
+                                               A.diag);
+      vRf = state == 0 ? v : vRf;
+      vLf = state == 1 ? v : vLf;
+      vRb = state == 2 ? v : vRb;
+      vLb = state == 3 ? v : vLb;
+    }
+#pragma unroll
+    for (int a = 4; a < 9; ++a) {
+      if (a == 8 && !BULK) {
+        cdv[4] = R(0);
+        continue;
+      }
+      const int n = a < 8 ? a : 14;
+      cdv[a - 4] = (ld(n, st) - ld(n, -st)) * hc;
+    }
+  }
+  // local propagation speeds, max(|v_L|, |v_R|) (LocalPropagationSpeed.cu:16-24)
+  const R rRf = R(vRf), rLf = R(vLf), rRb = R(vRb), rLb = R(vLb);
+  const R aF = fmax(fabs(rLf), fabs(rRf));
+  const R aB = fmax(fabs(rLb), fabs(rRb));
+
+  // directional sources: central differences of pi^{tau mu} and Pi and of
+  // the three spatial pi components of this direction
+  auto cd = [&](int n) { return (ldp(n, st) - ldp(n, -st)) * hc; };
+  const R cPi = cdv[4];
+  R Sd0, Sd1, Sd2, Sd3;
+  if constexpr (DIR == 0) {
+    Sd0 = cdv[0] * v_s - cdv[1] - (BULK ? v_s * cPi : R(0));
+    Sd1 = cdv[1] * v_s - cd(8) - cPi;
+    Sd2 = cdv[2] * v_s - cd(9);
+    Sd3 = cdv[3] * v_s - cd(10);
+  } else if constexpr (DIR == 1) {
+    Sd0 = cdv[0] * v_s - cdv[2] - (BULK ? v_s * cPi : R(0));
+    Sd2 = cdv[2] * v_s - cd(11) - cPi;
+    Sd1 = cdv[1] * v_s - cd(9);
+    Sd3 = cdv[3] * v_s - cd(12);
+  } else {
+    Sd0 = cdv[0] * v_s - cdv[3] - (BULK ? v_s * cPi : R(0));
+    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? cPi / R(t * t) : R(0));
+    Sd1 = cdv[1] * v_s - cd(10);
+    Sd2 = cdv[2] * v_s - cd(12);
+  }
+
+  const R ih = R(A.inv_h[DIR]);
+  const double dt = A.dt;
+  // H = (F(q_R) + F(q_L) - a (q_R - q_L))/2 with F = v q
+#define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
+  (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) * R(0.5) -                \
+    (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf))) * R(0.5)) * ih)
+  int n0 = 0;
+  if constexpr (KEEP_STATES) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      if (a == 8 && !BULK) continue;
+      const int n = a < 8 ? a : 14;
+      R r = GVH_FLUXDIFF(Wst[0][a], Wst[1][a], Wst[2][a], Wst[3][a]);
+      if (a == 0) r += Sd0;
+      if (a == 1) r += Sd1;
+      if (a == 2) r += Sd2;
+      if (a == 3) r += Sd3;
+      acc[n * BLOCK] += (double)r * dt;
+    }
+    n0 = 8;
+  }
+  const int n1 = KEEP_STATES ? 14 : NV;
+#pragma unroll 1
+  for (int n = n0; n < n1; ++n) {
+    const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
+            vp2 = ld(n, 2 * st);
+    const R D0 = slope_t<R>(vm1, v0, vp1);
+    const R rf = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
+    const R lf = v0 + D0 * R(0.5);
+    const R rb = v0 - D0 * R(0.5);
+    const R lb = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
+    R r = GVH_FLUXDIFF(rf, lf, rb, lb);
+    if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
+    acc[n * BLOCK] += (double)r * dt;
+  }
+#undef GVH_FLUXDIFF
+}
+
+// Cell of a thread.  Traversal order: x fastest, then the rows of a strip
+// of A.strip rows in y, then eta, then the strips -- so the eta neighbours
+// of a cell (+-2 planes) are a few strips-of-rows away in launch order rather
+// than a whole x-y plane, and stay in L2.  Blocks are renumbered so that each
+// of the 8 XCDs (which receive consecutive workgroups round-robin) gets one
+// contiguous run of that order.  The grid is a multiple of 8 blocks; threads
+// past the lattice (or in the padding rows of the last strip) exit.
+struct CellPos {
+  int i, j, k;
+  unsigned s;
+  bool valid;
+};
+
+// Cell of this thread in tile b of nb tiles (a multiple of 8): consecutive
+// tiles go to the 8 XCDs in turn, so tile b is renumbered to
+// lb = (b mod 8) nb/8 + b/8 and each XCD sweeps a contiguous range.
+__device__ __forceinline__ CellPos cell_of_tile(const StageArgs &A, unsigned b, unsigned nb) {
+  const unsigned lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  const int by = A.strip;
+  const long long tid = (long long)lb * BLOCK + threadIdx.x;
+  const int i = (int)(tid % A.nx);
+  const long long r = tid / A.nx;
+  const int jj = (int)(r % by);
+  const long long r2 = r / by;
+  const int k = (int)(r2 % A.nz);
+  const int yb = (int)(r2 / A.nz);
+  const int j = yb * by + jj;
+  CellPos c;
+  c.valid = j < A.ny && yb < A.nstrips;
+  c.i = i + NG;
+  c.j = (c.valid ? j : 0) + NG;
+  c.k = k + NG;
+  c.s = (unsigned)(c.i + (long long)A.ncx * (c.j + (long long)A.ncy * c.k));
+  return c;
+}
+
+__device__ __forceinline__ CellPos cell_of_thread(const StageArgs &A) {
+  return cell_of_tile(A, blockIdx.x, gridDim.x);
+}
+
+// Source part of the stage (eulerStepKernelSource_1D, EulerStep.cu:947-1009):
+// acc = q + dt S written to the stage's output buffer at the cell.
+template <int EOS, bool BULK, bool HAS_Z>
+__global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
+  constexpr int NV = BULK ? 15 : 14;
+  const CellPos c = cell_of_thread(A);
+  if (!c.valid) return;
+  const unsigned s = c.s;
+  const long long sy = A.ncx, sz = (long long)A.ncx * A.ncy;
+  const long long G = A.G;
+  const double *u = A.uin;
+  const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
+  const double e_s = A.ein[s], p_s = A.pin[s];
+  // minmod gradients of u and p (SourceTerms.cu:662-719)
+  Grad g;
+  const long long st[3] = {1, sy, sz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d == 2 && !HAS_Z) {
+      g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = 0.0;
+      continue;
+    }
+    const double ih = A.inv_h[d];
+    const double *um = u - st[d], *upl = u + st[d];
+    g.ut[d] = slope(um[s], ut, upl[s]) * ih;
+    g.ux[d] = slope((um + G)[s], ux, (upl + G)[s]) * ih;
+    g.uy[d] = slope((um + 2 * G)[s], uy, (upl + 2 * G)[s]) * ih;
+    g.un[d] = slope((um + 3 * G)[s], un, (upl + 3 * G)[s]) * ih;
+    g.p[d] = slope((A.pin - st[d])[s], p_s, (A.pin + st[d])[s]) * ih;
+  }
+  double Q[15], S[15];
+#pragma unroll
+  for (int n = 0; n < 15; ++n) Q[n] = (n < NV) ? (A.qin + n * G)[s] : 0.0;
+  S[14] = 0.0;
+  const double *up = A.uprev;
+  source_terms<EOS, BULK, double>(A, Q, e_s, e_s, p_s, ut, ux, uy, un, up[s], (up + G)[s],
+                                  (up + 2 * G)[s], (up + 3 * G)[s], g, S);
+#pragma unroll
+  for (int n = 0; n < NV; ++n) (A.qout + n * G)[s] = Q[n] + A.dt * S[n];
+}
+
+// Flux part of the stage plus the cell's finishing work: the three
+// directional updates in the reference's order, the convex average (stage
+// 2), the root-find, regulation, and the writes of the cell and its ghost
+// copies.  acc starts from what kt_source_kernel stored at the cell.
+// FUSED: the source terms are computed here first (one kernel per stage);
+// otherwise kt_source_kernel has stored acc = q + dt S at the cell.
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, bool LOCK,
+          typename R, typename PS>
+__device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c, double *acc) {
+  constexpr int NV = BULK ? 15 : 14;
+  const int i = c.i, j = c.j, k = c.k;
+  const unsigned s = c.s;
+  const long long sy = A.ncx, sz = (long long)A.ncx * A.ncy;
+  const long long G = A.G;
+  // conserved variable n of a buffer at this cell: T^{tau mu} in FP64, pi
+  // and Pi in the storage type PS
+  auto q_at = [&](const double *base, int n) -> double {
+    return n < 4 ? (base + n * G)[s]
+                 : (double)(reinterpret_cast<const PS *>(base + 4 * G) + (n - 4) * G)[s];
+  };
+  const double *u = A.uin;
+  const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
+  const double e_s = A.ein[s];
+  if constexpr (FUSED) {
+    const double p_s = A.pin[s];
+    GradT<R> g;
+    const long long st[3] = {1, sy, sz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d == 2 && !HAS_Z) {
+        g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = R(0);
+        continue;
+      }
+      const R ih = R(A.inv_h[d]);
+      const double *um = u - st[d], *upl = u + st[d];
+      g.ut[d] = slope_t<R>(R(um[s]), R(ut), R(upl[s])) * ih;
+      g.ux[d] = slope_t<R>(R((um + G)[s]), R(ux), R((upl + G)[s])) * ih;
+      g.uy[d] = slope_t<R>(R((um + 2 * G)[s]), R(uy), R((upl + 2 * G)[s])) * ih;
+      g.un[d] = slope_t<R>(R((um + 3 * G)[s]), R(un), R((upl + 3 * G)[s])) * ih;
+      g.p[d] = slope_t<R>(R((A.pin - st[d])[s]), R(p_s), R((A.pin + st[d])[s])) * ih;
+    }
+    double Qd[15];
+    R Q[15], S[15];
+#pragma unroll
+    for (int n = 0; n < 15; ++n) {
+      Qd[n] = (n < NV) ? q_at(A.qin, n) : 0.0;
+      Q[n] = R(Qd[n]);
+    }
+    S[14] = R(0);
+    const double *up = A.uprev;
+    source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un), R(up[s]),
+                               R((up + G)[s]), R((up + 2 * G)[s]), R((up + 3 * G)[s]), g, S);
+#pragma unroll
+    for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Qd[n] + A.dt * (double)S[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (A.qout + n * G)[s];
+  }
+
+  // flux velocity of the cell itself for the directional sources (v = u/u^tau)
+  direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, R(ux / ut), acc);
+  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS>(A, s, e_s, R(uy / ut), acc);
+  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, R(un / ut), acc);
+
+  if constexpr (STAGE2) {
+#pragma unroll 1
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + q_at(A.qn, n)) * 0.5;
+  }
+
+  // inferred variables with this cell's e as the guess
+  const Prim pr = inferred<EOS>(A.tab, A.tinf, acc[0], acc[BLOCK], acc[2 * BLOCK], acc[3 * BLOCK],
+                                acc[4 * BLOCK], acc[5 * BLOCK], acc[6 * BLOCK], acc[7 * BLOCK],
+                                BULK ? acc[14 * BLOCK] : 0.0, e_s, A.diag);
+
+  // regulateDissipativeCurrents (RegulateDissipativeCurrents.cu:16-134): only
+  // cells at or below e_reg are rescaled; the bulk factor is 1 unless p is 0
+  if (!(pr.e > A.e_reg)) {
+    const double t = A.tinf, t2 = t * t;
+    const double pitt = acc[4 * BLOCK], pitx = acc[5 * BLOCK], pity = acc[6 * BLOCK],
+                 pitn = acc[7 * BLOCK], pixx = acc[8 * BLOCK], pixy = acc[9 * BLOCK],
+                 pixn = acc[10 * BLOCK], piyy = acc[11 * BLOCK], piyn = acc[12 * BLOCK],
+                 pinn = acc[13 * BLOCK];
+    const double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx +
+                        2 * pixy * pixy + piyy * piyy - 2 * pitn * pitn * t2 +
+                        2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 + pinn * pinn * t2 * t2;
+    const double spipi = sqrt(fabs(pipi));
+    const double pimumu = pitt - pixx - piyy - pinn * t2;
+    const double piu0 = -pitn * t2 * pr.un + pitt * pr.ut - pitx * pr.ux - pity * pr.uy;
+    const double piu1 = -pixn * t2 * pr.un + pitx * pr.ut - pixx * pr.ux - pixy * pr.uy;
+    const double piu2 = -piyn * t2 * pr.un + pity * pr.ut - pixy * pr.ux - piyy * pr.uy;
+    const double piu3 = -pinn * t2 * pr.un + pitn * pr.ut - pixn * pr.ux - piyn * pr.uy;
+    const double a1 = spipi * (1.0 / sqrt(pr.e * pr.e + 3 * pr.p * pr.p));  // rho_max = 1
+    const double den = 0.1 * spipi;                                          // xi0 rho_max
+    const double rho = fmax(a1, fmax(pimumu / den, fmax(piu0 / den, fmax(piu1 / den,
+                       fmax(piu2 / den, piu3 / den)))));
+    double fs = tanh(rho) / rho;
+    if (fabs(rho) < 1.e-7) fs = 1.0;
+#pragma unroll 1
+    for (int n = 4; n < 14; ++n) acc[n * BLOCK] *= fs;
+    if constexpr (BULK) {
+      const double rb = 0.0 / pr.p;
+      double fb = tanh(rb) / rb;
+      if (fabs(rb) < 1.e-7) fb = 1.0;
+      acc[14 * BLOCK] *= fb;
+    }
+  }
+
+  // write the cell and, on the boundary, its two ghost layers
+  // (GhostCells.cu:15-144: zero-gradient copy, edges and corners untouched)
+  // offsets of this cell's ghost copies: two per lattice face it touches (a
+  // cell of a one-point axis touches both faces of that axis)
+  long long gh[12];
+  int ng = 0;
+  if (i == NG) { gh[ng++] = -1; gh[ng++] = -2; }
+  if (i == A.nx + 1) { gh[ng++] = 1; gh[ng++] = 2; }
+  if (j == NG) { gh[ng++] = -sy; gh[ng++] = -2 * sy; }
+  if (j == A.ny + 1) { gh[ng++] = sy; gh[ng++] = 2 * sy; }
+  if (k == NG) { gh[ng++] = -sz; gh[ng++] = -2 * sz; }
+  if (k == A.nz + 1) { gh[ng++] = sz; gh[ng++] = 2 * sz; }
+
+  auto put = [&](auto *base, auto val) {
+    base[s] = val;
+    for (int m = 0; m < ng; ++m) base[(long long)s + gh[m]] = val;
+  };
+  PS *qoutP = reinterpret_cast<PS *>(A.qout + 4 * G);
+#pragma unroll 1
+  for (int n = 0; n < 4; ++n) put(A.qout + n * G, acc[n * BLOCK]);
+#pragma unroll 1
+  for (int n = 4; n < NV; ++n) put(qoutP + (n - 4) * G, (PS)acc[n * BLOCK]);
+  put(A.eout, pr.e);
+  put(A.pout, pr.p);
+  put(A.uout, pr.ut);
+  put(A.uout + G, pr.ux);
+  put(A.uout + 2 * G, pr.uy);
+  put(A.uout + 3 * G, pr.un);
+}
+
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, bool LOCK,
+          typename R, typename PS>
+__global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
+  __shared__ double acc_lds[15 * BLOCK];
+  const CellPos c = cell_of_thread(A);
+  if (!c.valid) return;
+  stage_cell<EOS, BULK, STAGE2, HAS_Z, FUSED, KEEP, LOCK, R, PS>(A, c, acc_lds + threadIdx.x);
+}
+
+}  // namespace gvh

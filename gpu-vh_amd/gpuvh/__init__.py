@@ -96,6 +96,23 @@ class RunStats(C.Structure):
                 ("ms_freezeout", C.c_double), ("ms_total", C.c_double)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("hbm_bytes", C.c_ulonglong), ("max_events", C.c_int), ("fofreq", C.c_int),
+                ("autotune", C.c_int)]
+
+
+class Event(C.Structure):
+    _fields_ = [("lattice", LatticeParameters), ("hydro", HydroParameters),
+                ("e", C.POINTER(C.c_double)), ("surface", C.c_int), ("fo_flags", C.c_int),
+                ("keep_state", C.c_int)]
+
+
+class EventResult(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("steps", C.c_int), ("frozen", C.c_int),
+                ("t_final", C.c_double), ("surface_rows", C.c_longlong),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
 class Diagnostics(C.Structure):
     _fields_ = [("newton_not_converged", C.c_ulonglong), ("e_nan", C.c_ulonglong),
                 ("e_clamped", C.c_ulonglong)]
@@ -149,6 +166,27 @@ def lib():
     }
     L.gvh_run_options_default.argtypes = [C.POINTER(RunOptions)]
     L.gvh_run_options_default.restype = None
+    L.gvh_batch_options_default.argtypes = [C.POINTER(BatchOptions)]
+    L.gvh_batch_options_default.restype = None
+    bsig = {
+        "gvh_batch_create": [C.POINTER(Options), C.POINTER(BatchOptions), C.POINTER(C.c_void_p)],
+        "gvh_batch_destroy": [C.c_void_p],
+        "gvh_batch_submit": [C.c_void_p, C.POINTER(Event), C.POINTER(C.c_longlong)],
+        "gvh_batch_step": [C.c_void_p, C.POINTER(C.c_int)],
+        "gvh_batch_run": [C.c_void_p],
+        "gvh_batch_poll": [C.c_void_p, C.POINTER(EventResult), C.c_int, C.POINTER(C.c_int)],
+        "gvh_batch_rows": [C.c_void_p, C.c_longlong, dp, C.c_longlong],
+        "gvh_batch_field": [C.c_void_p, C.c_longlong, C.c_int, dp],
+        "gvh_batch_release": [C.c_void_p, C.c_longlong],
+        "gvh_batch_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                            C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)],
+        "gvh_autotune": [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                         C.POINTER(C.c_float)],
+    }
+    for name, args in bsig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -358,6 +396,13 @@ class Context:
             _check(lib().gvh_freezeout_rows(self.h, _ptr(rows), cnt.value))
         return rows
 
+    def autotune(self, steps=3):
+        """Time the kernel arrangements on the current state (restored) and
+        keep the fastest; returns (variant, strip, ms per step)."""
+        v, st, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().gvh_autotune(self.h, steps, C.byref(v), C.byref(st), C.byref(ms)))
+        return v.value, st.value, ms.value
+
     def freezeout_slot(self, logical, var):
         out = np.empty(self.nx * self.ny * self.nz, dtype=np.float64)
         _check(lib().gvh_freezeout_slot(self.h, logical, var, _ptr(out)))
@@ -395,3 +440,75 @@ def run(lattice_params, ic_params, hydro_params, root_dir, output_dir, opts, ro=
                          root_dir.encode(), output_dir.encode(), C.byref(opts),
                          C.byref(ro) if ro is not None else None, C.byref(st)))
     return {f: getattr(st, f) for f, _ in RunStats._fields_}
+
+
+class Batch:
+    """Many events advancing together on the device (include/gpuvh.h, batches)."""
+
+    def __init__(self, opts, hbm_bytes=0, max_events=64, fofreq=10, autotune=True):
+        bo = BatchOptions()
+        lib().gvh_batch_options_default(C.byref(bo))
+        bo.hbm_bytes, bo.max_events, bo.fofreq, bo.autotune = hbm_bytes, max_events, fofreq, int(autotune)
+        self.opts = opts
+        h = C.c_void_p()
+        _check(lib().gvh_batch_create(C.byref(opts), C.byref(bo), C.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if self.h:
+            lib().gvh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, lattice_params, hydro_params, e, surface=False, fo_flags=0, keep_state=False):
+        ev = Event()
+        ev.lattice, ev.hydro = lattice_params, hydro_params
+        a = Context._flat(e)
+        ev.e = _ptr(a)
+        ev.surface, ev.fo_flags, ev.keep_state = int(surface), fo_flags, int(keep_state)
+        i = C.c_longlong()
+        _check(lib().gvh_batch_submit(self.h, C.byref(ev), C.byref(i)))
+        self.shapes[i.value] = (lattice_params.numLatticePointsX, lattice_params.numLatticePointsY,
+                                lattice_params.numLatticePointsRapidity)
+        return i.value
+
+    def step(self):
+        n = C.c_int()
+        _check(lib().gvh_batch_step(self.h, C.byref(n)))
+        return n.value
+
+    def run(self):
+        _check(lib().gvh_batch_run(self.h))
+
+    def poll(self, max_results=1024):
+        out = (EventResult * max_results)()
+        n = C.c_int()
+        _check(lib().gvh_batch_poll(self.h, out, max_results, C.byref(n)))
+        return [{f: getattr(out[k], f) for f, _ in EventResult._fields_} for k in range(n.value)]
+
+    def rows(self, event_id, nrows):
+        r = np.empty((nrows, FO_COLUMNS), dtype=np.float64)
+        if nrows:
+            _check(lib().gvh_batch_rows(self.h, event_id, _ptr(r), nrows))
+        return r
+
+    def field(self, event_id, field):
+        nx, ny, nz = self.shapes[event_id]
+        out = np.empty(nx * ny * nz, dtype=np.float64)
+        _check(lib().gvh_batch_field(self.h, event_id, field, _ptr(out)))
+        return out.reshape((nx, ny, nz), order="F")
+
+    def release(self, event_id):
+        _check(lib().gvh_batch_release(self.h, event_id))
+
+    def stats(self):
+        q, r = C.c_longlong(), C.c_longlong()
+        u, b = C.c_ulonglong(), C.c_ulonglong()
+        _check(lib().gvh_batch_stats(self.h, C.byref(q), C.byref(r), C.byref(u), C.byref(b)))
+        return {"queued": q.value, "running": r.value, "hbm_used": u.value, "hbm_budget": b.value}

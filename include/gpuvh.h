@@ -265,6 +265,69 @@ int gvh_run(const gvh_lattice_parameters *lattice, const gvh_ic_parameters *ic,
  * directory, root directory = working directory. */
 int gvh_run_hydro(int argc, char **argv, const gvh_options *opts, const gvh_run_options *ro);
 
+/* ------------------------------------------------------------- batches
+ * Many events on one GPU with continuous batching.  No counterpart in the
+ * reference, which runs one event per process (HydroPlugin.cpp:107-434):
+ * each event here follows that run loop (nt + 1 steps at most, the
+ * all-frozen stop of :396-413, freeze-out windows of :270-393) while all
+ * running events advance together, one launch per kernel per step, from a
+ * captured HIP graph.  Events share the batch's gvh_options (EoS, bulk,
+ * precision); each has its own lattice, time step and hydro parameters. */
+typedef struct gvh_batch gvh_batch;
+
+typedef struct gvh_batch_options {
+  unsigned long long hbm_bytes; /* admission budget; 0 = 80% of free device memory */
+  int max_events;               /* event slots (concurrently running or kept)      */
+  int fofreq;                   /* freeze-out window FOFREQ, reference 10           */
+  int autotune;                 /* time kernel arrangements per lattice shape       */
+} gvh_batch_options;
+
+typedef struct gvh_event {
+  gvh_lattice_parameters lattice; /* numProperTimePoints = the run loop's nt */
+  gvh_hydro_parameters hydro;     /* tau0, eta/s, T_fo, T_reg                */
+  const double *e;                /* initial energy density, nx*ny*nz, x fastest;
+                                     at rest, pi = Pi = 0, p from the EoS     */
+  int surface;                    /* find the freeze-out surface              */
+  int fo_flags;                   /* GVH_FO_* for the surface rows            */
+  int keep_state;                 /* keep the final state for gvh_batch_field */
+} gvh_event;
+
+typedef struct gvh_event_result {
+  long long id;
+  int steps;                /* KT steps taken                              */
+  int frozen;               /* 1 if ended by the all-frozen rule           */
+  double t_final;           /* proper time at the end                      */
+  long long surface_rows;   /* rows found (gvh_batch_rows)                 */
+  double t_submit, t_admit, t_done; /* seconds on the batch's clock        */
+} gvh_event_result;
+
+void gvh_batch_options_default(gvh_batch_options *bo);
+int gvh_batch_create(const gvh_options *opts, const gvh_batch_options *bo, gvh_batch **out);
+int gvh_batch_destroy(gvh_batch *b);
+/* Queue an event (its arrays are copied). */
+int gvh_batch_submit(gvh_batch *b, const gvh_event *ev, long long *id);
+/* Admit queued events that fit, advance every running event by one step,
+ * search due freeze-out windows, retire finished events.  *nrunning =
+ * running + queued events afterwards. */
+int gvh_batch_step(gvh_batch *b, int *nrunning);
+/* gvh_batch_step until nothing is queued or running. */
+int gvh_batch_run(gvh_batch *b);
+/* Up to max results of events finished since the last poll. */
+int gvh_batch_poll(gvh_batch *b, gvh_event_result *out, int max, int *n);
+/* Surface rows (GVH_FO_COLUMNS doubles each) of a finished event. */
+int gvh_batch_rows(gvh_batch *b, long long id, double *rows, long long nrows);
+/* Interior field of a finished event submitted with keep_state. */
+int gvh_batch_field(gvh_batch *b, long long id, int field, double *host_out);
+/* Forget a finished event (frees a kept state). */
+int gvh_batch_release(gvh_batch *b, long long id);
+int gvh_batch_stats(gvh_batch *b, long long *queued, long long *running,
+                    unsigned long long *hbm_used, unsigned long long *hbm_budget);
+
+/* Time the stage-kernel arrangements and traversal strips on the context's
+ * current state (restored afterwards) and keep the fastest; no counterpart
+ * in the reference (fixed launch shapes, CudaConfiguration.cu:27-28). */
+int gvh_autotune(gvh_context *ctx, int steps, int *variant, int *strip, float *ms_per_step);
+
 const char *gvh_last_error(void);
 
 #ifdef __cplusplus

@@ -298,6 +298,8 @@ static double energy_from_conserved(const kto_params *P, double ePrev,
       return e;
     }
     e0 = e;
+    /* a NaN iterate stays NaN to MAX_ITERS and is returned: same value */
+    if (isnan(e0)) break;
   }
   g_last_iters = MAX_ITERS;
   return e0;
