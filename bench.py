@@ -108,7 +108,8 @@ def measured_traffic(workload_key):
 def mixed_error(args, e, stream, nsteps):
     """Max relative error of the mixed form against FP64 after nsteps on the
     same event: per cell with a floor of 1% of the field's maximum (u^x, u^y,
-    u^eta and pi cross zero), and the Lorentz-norm error max|du|/u^tau."""
+    u^eta and pi cross zero), and the Lorentz-norm error max|du|/u^tau.
+    Cells where either form is not finite are counted, not compared."""
     import gpuvh
 
     ctxs = {}
@@ -123,11 +124,15 @@ def mixed_error(args, e, stream, nsteps):
               "pi": [gpuvh.PITT, gpuvh.PITX, gpuvh.PITY, gpuvh.PITN, gpuvh.PIXX, gpuvh.PIXY,
                      gpuvh.PIXN, gpuvh.PIYY, gpuvh.PIYN, gpuvh.PINN], "Pi": [gpuvh.PI]}
     du = 0.0
+    nonfinite = 0
     for name, fields in groups.items():
         worst = 0.0
         for f in fields:
             a, b = ctxs[gpuvh.FP64].get(f), ctxs[gpuvh.MIXED].get(f)
-            scale = np.abs(a).max()
+            ok = np.isfinite(a) & np.isfinite(b)
+            nonfinite = max(nonfinite, int((~ok).sum()))
+            a, b = a[ok], b[ok]
+            scale = float(np.abs(a).max()) if a.size else 0.0
             if scale == 0:
                 continue
             worst = max(worst, float((np.abs(b - a) / np.maximum(np.abs(a), 1e-2 * scale)).max()))
@@ -135,12 +140,170 @@ def mixed_error(args, e, stream, nsteps):
                 du = max(du, float(np.abs(b - a).max()))
         out[name] = worst
     ut = ctxs[gpuvh.FP64].get(gpuvh.UT)
-    out["u_lorentz"] = du / float(np.abs(ut).min())
+    out["u_lorentz"] = du / float(np.nanmin(np.abs(ut)))
+    out["nonfinite_cells"] = nonfinite
     out["diagnostics_fp64"] = ctxs[gpuvh.FP64].diagnostics()
     out["diagnostics_mixed"] = ctxs[gpuvh.MIXED].diagnostics()
     for c in ctxs.values():
         c.close()
     return out
+
+
+def batch_workload(n_events, seed, kind):
+    """Seeded events of the batched (kind "batch": 2+1D, transverse grids odd
+    121..261, 300..800 steps, 50..300 hotspots) or serving (kind "serve": half
+    2+1D 121^2..201^2, half 3+1D 121^2..201^2 x 21..51, through freeze-out at
+    T_fo = 0.15 GeV) workloads; event k uses seed + k."""
+    import gpuvh
+    from gpuvh import ic
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_events):
+        if kind == "batch":
+            nx = 2 * int(rng.integers(60, 131)) + 1
+            nz, nt = 1, int(rng.integers(300, 801))
+            Tfo = 0.0  # fixed step count: never frozen
+        else:
+            nx = 2 * int(rng.integers(60, 101)) + 1
+            nz = 1 if k % 2 == 0 else 2 * int(rng.integers(10, 26)) + 1
+            nt, Tfo = 4000, 0.15
+        nh = int(rng.integers(50, 301))
+        e = ic.serving_event(nx, nx, nz, 0.1, 0.1, seed + k, nh)
+        L = gpuvh.lattice(nx, nx, nz, 0.1, 0.1, 0.1, 0.01, nt=nt)
+        H = gpuvh.hydro(0.5, 0.16, T_fo_GeV=Tfo)
+        out.append((L, H, e))
+    return out
+
+
+def run_batch_mode(args):
+    """64 2+1D events in one batch (mixed precision, autotuned) against the
+    same events one by one in plain FP64: throughput and error."""
+    import torch
+    import gpuvh
+
+    table = gpuvh.EosTable()
+    events = batch_workload(args.events, args.seed, "batch")
+    cells = sum(L.numLatticePointsX * L.numLatticePointsY * (L.numProperTimePoints + 1)
+                for L, H, e in events)
+    log("batch: %d events, %.3g cell-updates" % (len(events), cells))
+    res = {}
+    keep = min(4, len(events))
+    kept = {}
+    for prec, name in ((gpuvh.MIXED, "mixed"), (gpuvh.FP64, "fp64")):
+        opts = gpuvh.options(gpuvh.EOS_TABLE, True, prec, 0.16 / gpuvh.HBARC, table=table)
+        b = gpuvh.Batch(opts, max_events=len(events), autotune=True)
+        ids = [b.submit(L, H, e, keep_state=(k < keep)) for k, (L, H, e) in enumerate(events)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run()
+        el = time.perf_counter() - t0
+        res["batch_" + name] = el
+        kept[name] = [{f: b.field(ids[k], f) for f in (gpuvh.E, gpuvh.UX, gpuvh.UY)}
+                      for k in range(keep)]
+        b.close()
+        log("batch %s: %.3f s, %.3g cell-updates/s" % (name, el, cells / el))
+    # one by one in plain FP64 on contexts of their own
+    opts = gpuvh.options(gpuvh.EOS_TABLE, True, gpuvh.FP64, 0.16 / gpuvh.HBARC, table=table)
+    from gpuvh import ic
+    el = 0.0
+    alone = []
+    for k, (L, H, e) in enumerate(events):
+        ctx = gpuvh.Context(L, H, opts)
+        ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+        ctx.set_initial_state(0.5, e, ctx.equilibrium_pressure(e), ut, ux, uy, un)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.steps(0.5, 0, L.numProperTimePoints + 1)
+        ctx.synchronize()
+        el += time.perf_counter() - t0
+        if k < keep:
+            alone.append({f: ctx.get(f) for f in (gpuvh.E, gpuvh.UX, gpuvh.UY)})
+        ctx.close()
+    res["alone_fp64"] = el
+    log("one by one fp64: %.3f s, %.3g cell-updates/s" % (el, cells / el))
+    errs = {"e": 0.0, "u": 0.0}
+    for k in range(keep):
+        for f, g in ((gpuvh.E, "e"), (gpuvh.UX, "u"), (gpuvh.UY, "u")):
+            a, m = alone[k][f], kept["mixed"][k][f]
+            den = np.maximum(np.abs(a), 1e-2 * np.abs(a).max())
+            errs[g] = max(errs[g], float(np.max(np.abs(m - a) / den)))
+    out = {"metric": "cell-updates/s, 64 batched 2+1D events", "value": round(cells / res["batch_mixed"], 1),
+           "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32/f64 mixed",
+           "data": "synthetic",
+           "config": {"workload": "%d 2+1D events, transverse grids 121..261 (odd), dx 0.1, 300..800 "
+                                  "steps, 50..300 hotspots on a Gaussian base, tabulated EoS, "
+                                  "shear+bulk, autotuned" % len(events), "seed": args.seed},
+           "events_per_s": round(len(events) / res["batch_mixed"], 2),
+           "batch_mixed_s": round(res["batch_mixed"], 3), "batch_fp64_s": round(res["batch_fp64"], 3),
+           "one_by_one_fp64_s": round(res["alone_fp64"], 3),
+           "speedup_vs_one_by_one_fp64": round(res["alone_fp64"] / res["batch_mixed"], 3),
+           "batch_fp64_cell_updates_per_s": round(cells / res["batch_fp64"], 1),
+           "mixed_error_vs_fp64_alone": {k: float("%.3e" % v) for k, v in errs.items()},
+           "error_metric": "max per-cell |mixed - fp64| / max(|fp64|, 1e-2 max|fp64|) over the "
+                           "first %d events at their last step" % keep}
+    print(json.dumps(out))
+
+
+def run_serve_mode(args):
+    """Serving loop: a Poisson stream of events at 80% of the measured
+    capacity into a continuously batched engine; events/s and latency."""
+    import gpuvh
+
+    table = gpuvh.EosTable()
+    events = batch_workload(args.events, args.seed, "serve")
+    report = {}
+    for prec, name in ((gpuvh.MIXED, "mixed"), (gpuvh.FP64, "fp64")):
+        opts = gpuvh.options(gpuvh.EOS_TABLE, True, prec, 0.16 / gpuvh.HBARC, table=table)
+        # capacity: everything submitted at once
+        b = gpuvh.Batch(opts, max_events=len(events), autotune=True)
+        for (L, H, e) in events:
+            b.submit(L, H, e, surface=True)
+        t0 = time.perf_counter()
+        b.run()
+        cap_s = time.perf_counter() - t0
+        res = b.poll(len(events))
+        b.close()
+        capacity = len(events) / cap_s
+        log("serve %s: capacity %.2f events/s (%d events in %.2f s)" % (name, capacity, len(res), cap_s))
+        # Poisson arrivals at 80% of capacity
+        rate = 0.8 * capacity
+        rng = np.random.default_rng(args.seed + 1)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(events)))
+        b = gpuvh.Batch(opts, max_events=len(events), autotune=True)
+        start = time.perf_counter()
+        nsub, done = 0, []
+        while len(done) < len(events):
+            now = time.perf_counter() - start
+            while nsub < len(events) and arrivals[nsub] <= now:
+                L, H, e = events[nsub]
+                b.submit(L, H, e, surface=True)
+                nsub += 1
+            running = b.step()
+            done += b.poll(len(events))
+            if running == 0 and nsub < len(events):
+                time.sleep(max(0.0, arrivals[nsub] - (time.perf_counter() - start)))
+        makespan = time.perf_counter() - start
+        b.close()
+        lat = np.array([r["t_done"] - r["t_submit"] for r in done])
+        rows = int(sum(r["surface_rows"] for r in done))
+        report[name] = {"capacity_events_per_s": round(capacity, 3), "offered_rate": round(rate, 3),
+                        "events_per_s": round(len(done) / makespan, 3),
+                        "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+                        "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+                        "latency_mean_s": round(float(lat.mean()), 3),
+                        "steps_mean": round(float(np.mean([r["steps"] for r in done])), 1),
+                        "frozen": int(sum(r["frozen"] for r in done)), "surface_rows": rows}
+        log("serve %s: %s" % (name, report[name]))
+    out = {"metric": "events/s at 80%% of capacity, p50/p99 latency", "value": report["mixed"]["events_per_s"],
+           "unit": "events/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32/f64 mixed",
+           "data": "synthetic",
+           "config": {"workload": "%d events, half 2+1D 121^2..201^2, half 3+1D 121^2..201^2 x 21..51, "
+                                  "hotspots on a Gaussian base, tabulated EoS, shear+bulk, through "
+                                  "freeze-out at T_fo 0.15 GeV with surfaces, Poisson arrivals"
+                                  % len(events), "seed": args.seed},
+           "mixed": report["mixed"], "fp64": report["fp64"]}
+    print(json.dumps(out))
 
 
 def cpu_baseline(args, e_full, threads):
@@ -188,12 +351,17 @@ def main():
                     help="stage-kernel arrangement (library default if unset)")
     ap.add_argument("--timed-steps", type=int, default=5,
                     help="extra steps with HIP events around each stage kernel")
-    ap.add_argument("--error-steps", type=int, default=100,
+    ap.add_argument("--error-steps", type=int, default=50,
                     help="steps after which mixed is compared with FP64 (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision report")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="batched 2+1D events against one-by-one FP64 (prints its own line)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving loop with Poisson arrivals (prints its own line)")
+    ap.add_argument("--events", type=int, default=64, help="events of --batch / --serve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -212,6 +380,12 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    if args.batch:
+        run_batch_mode(args)
+        return
+    if args.serve:
+        run_serve_mode(args)
+        return
 
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -249,10 +423,14 @@ def main():
     head = res[args.precision]
     wkey = "%dx%dx%d/%s/%s/v%s" % (args.nx, args.ny, args.nz, args.eos, args.precision,
                                    os.environ.get("GVH_VARIANT", "default"))
-    traffic = measured_traffic(wkey)
+    rec = measured_traffic(wkey)
     roofline = {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
+                "traffic": rec["bytes_per_step"] if rec else None,
+                "traffic_unit": "HBM bytes per KT step (the two stage launches), rocprofv3 "
+                                "FETCH_SIZE x fetch_factor + WRITE_SIZE x write_factor",
+                "traffic_stage_launch_bytes": rec["stage_bytes"] if rec else None,
+                "traffic_source": rec["source"] if rec else None,
                 "kernel": "kt_flux_kernel (one launch per RK stage)",
                 "stage_ms": [round(v, 4) for v in head["stage_ms"]],
                 "bytes_per_cell_update": head["bytes_per_cell_update"],

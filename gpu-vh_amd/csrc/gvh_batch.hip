@@ -124,7 +124,7 @@ struct gvh_batch {
   hipGraphExec_t exec = nullptr;
   long long next_id = 0;
   Clock::time_point origin;
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;  // shape -> (variant, strip)
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;  // shape class -> (variant, strip)
 };
 
 namespace {
@@ -285,7 +285,9 @@ int admit(gvh_batch *b) {
     rc = gvh_set_initial_state(c, t0, P.e.data(), p.data(), ut.data(), zero.data(), zero.data(),
                                zero.data(), nullptr, nullptr);
     if (!rc && b->autotune) {
-      const auto key = std::make_tuple(c->nx, c->ny, c->nz);
+      // one tuning per shape class: 2+1D or 3+1D, y extent in steps of 40,
+      // eta extent in steps of 16
+      const auto key = std::make_tuple(c->nz > 1 ? 1 : 0, c->ny / 40, c->nz / 16);
       auto it = b->tuned.find(key);
       if (it == b->tuned.end()) {
         int v = 0, st = 16;

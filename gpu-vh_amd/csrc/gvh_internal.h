@@ -82,7 +82,7 @@ struct gvh_context {
   int iu = 0;   // buffer holding u^n
   int iup = 1;  // buffer holding u^{n-1}
   int ie = 0;   // (e, p) buffer holding the current state
-  int variant = 0;  // stage kernel arrangement (gvh_stage.hip launch_z)
+  int variant = 3;  // stage kernel arrangement (gvh_stage.hip launch_z)
   int strip = 16;   // y rows per traversal strip
   bool initialized = false;
   long long steps_done = 0;     // KT steps since the initial state was set

@@ -105,6 +105,22 @@ def c2_event(nx=201, ny=201, nz=101, dx=0.1, dy=0.1, dz=0.1, seed=2,
     return quantise_like_reference(eT[:, :, None] * eL[None, None, :])
 
 
+def serving_event(nx, ny, nz, dx, dz, seed, n_hot, base_peak=15.0, hot_amp_max=5.0):
+    """Event of the batched and serving workloads: a transverse Gaussian base
+    (base_peak GeV/fm^3, sigma 0.45 of the lattice half-width, so no cell
+    starts near vacuum, where viscous evolution without regulation breaks
+    down within ~100-200 steps) plus n_hot seeded hotspots (sigma 0.5 fm,
+    amplitude U(0, hot_amp_max) GeV/fm^3, centres in the central half-width),
+    times the plateau-plus-tail eta profile for nz > 1.  Shape (nx, ny, nz),
+    fm^-4, quantised like the reference's file reader."""
+    half = 0.5 * (min(nx, ny) - 1) * dx
+    eT = hotspot_event_2d(nx, ny, dx, dx, seed, n_hot=n_hot, hot_sigma=0.5,
+                          hot_amp_max=hot_amp_max, base_peak=base_peak,
+                          base_sigma=0.45 * half, centre_radius=0.5 * half)
+    eL = longitudinal_profile(nz, dz) if nz > 1 else np.ones(1)
+    return quantise_like_reference(eT[:, :, None] * eL[None, None, :])
+
+
 def at_rest(shape):
     """u^mu = (1,0,0,0) and pi^{mu nu} = 0, Pi = 0."""
     ut = np.ones(shape)
