@@ -9,9 +9,10 @@ namespace gvh {
 // Kernel arrangement of a stage (StageArgs::variant): 0 fused kernel with
 // the inversion states kept in registers, 1 fused with states rebuilt one at
 // a time, 2 separate source kernel + flux kernel (states rebuilt), 3 as 0
-// with the four root-finds of a direction in lockstep.  The mixed form
+// with the four root-finds of a direction in lockstep, 4 as 3 with the
+// states rebuilt for the flux after the root-finds.  The mixed form
 // (FP32 stencil, limiter, flux and sources, FP32 pi/Pi storage, FP64
-// T^{tau mu}, accumulator, root-finds and EoS) has arrangements 0 and 3;
+// T^{tau mu}, accumulator, root-finds and EoS) has arrangements 0, 3 and 4;
 // others run as 0.
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool MIXED>
 static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
@@ -21,34 +22,42 @@ static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
   if constexpr (MIXED) {
     // FP32 stencil, limiter, flux and sources; FP32 pi/Pi storage
     if (A.variant == 3)
-      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, true, float, float>),
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 1, float, float>),
+                         dim3(grid), dim3(block), 0, st, A);
+    else if (A.variant == 4)
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 2, float, float>),
                          dim3(grid), dim3(block), 0, st, A);
     else
-      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, false, float, float>),
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 0, float, float>),
                          dim3(grid), dim3(block), 0, st, A);
     return hipGetLastError();
   } else {
     switch (A.variant) {
       case 0:
         hipLaunchKernelGGL(
-            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, false, double, double>),
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 0, double, double>),
             dim3(grid), dim3(block), 0, st, A);
         break;
       case 1:
         hipLaunchKernelGGL(
-            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, false, false, double, double>),
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, false, 0, double, double>),
             dim3(grid), dim3(block), 0, st, A);
         break;
       case 3:
         hipLaunchKernelGGL(
-            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, true, double, double>),
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 1, double, double>),
+            dim3(grid), dim3(block), 0, st, A);
+        break;
+      case 4:
+        hipLaunchKernelGGL(
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 2, double, double>),
             dim3(grid), dim3(block), 0, st, A);
         break;
       default:
         hipLaunchKernelGGL((kt_source_kernel<EOS, BULK, HAS_Z>), dim3(grid), dim3(block), 0, st,
                            A);
         hipLaunchKernelGGL(
-            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, false, false, false, double, double>),
+            (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, false, false, 0, double, double>),
             dim3(grid), dim3(block), 0, st, A);
     }
     return hipGetLastError();

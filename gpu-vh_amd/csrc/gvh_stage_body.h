@@ -24,6 +24,23 @@
 #include "../../include/gpuvh.h"
 #include "gvh_device.h"
 
+// Waves per SIMD asked of the one-event stage kernel (the second launch
+// bound is amdgpu_waves_per_eu; 1 leaves the register allocation free).
+// Measured on the 201x201x101 table-EoS event: the FP32 (mixed) form at three
+// waves with a few spills runs 13% faster than at its free two; the FP64 form
+// runs 5% slower when held to two than when free (same register count,
+// different schedule), and at three its spills cost more than the wave gains.
+#ifndef GVH_WAVES_FP32
+#define GVH_WAVES_FP32 3
+#endif
+#ifndef GVH_WAVES_FP64
+#define GVH_WAVES_FP64 1
+#endif
+template <typename R>
+constexpr int stage_waves_per_simd() {
+  return sizeof(R) == 4 ? GVH_WAVES_FP32 : GVH_WAVES_FP64;
+}
+
 namespace gvh {
 
 // ------------------------------------------------------ source terms
@@ -257,7 +274,7 @@ constexpr int BLOCK = 256;
 // the four solves, then every variable once more for its fluxes.
 // Every stencil load is a uniform (scalar) base plus the thread's 32-bit
 // cell offset.
-template <int EOS, bool BULK, int DIR, bool KEEP_STATES, bool LOCK, typename R, typename PS>
+template <int EOS, bool BULK, int DIR, bool KEEP_STATES, int LOCK, typename R, typename PS>
 __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s, double ePrev,
                                                  R v_s, double *acc) {
   constexpr int NV = BULK ? 15 : 14;
@@ -381,8 +398,12 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
 #define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
   (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) * R(0.5) -                \
     (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf))) * R(0.5)) * ih)
+  // the kept states give the flux of the nine inversion variables, unless
+  // LOCK == 2, which rebuilds every state from the stencil after the
+  // root-finds so that none stays live through them
+  constexpr bool FLUX_FROM_KEPT = KEEP_STATES && LOCK != 2;
   int n0 = 0;
-  if constexpr (KEEP_STATES) {
+  if constexpr (FLUX_FROM_KEPT) {
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       if (a == 8 && !BULK) continue;
@@ -396,7 +417,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     }
     n0 = 8;
   }
-  const int n1 = KEEP_STATES ? 14 : NV;
+  const int n1 = FLUX_FROM_KEPT ? 14 : NV;
 #pragma unroll 1
   for (int n = n0; n < n1; ++n) {
     const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
@@ -500,7 +521,7 @@ __global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
 // copies.  acc starts from what kt_source_kernel stored at the cell.
 // FUSED: the source terms are computed here first (one kernel per stage);
 // otherwise kt_source_kernel has stored acc = q + dt S at the cell.
-template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, bool LOCK,
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, int LOCK,
           typename R, typename PS>
 __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c, double *acc) {
   constexpr int NV = BULK ? 15 : 14;
@@ -631,9 +652,9 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   put(A.uout + 3 * G, pr.un);
 }
 
-template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, bool LOCK,
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, int LOCK,
           typename R, typename PS>
-__global__ __launch_bounds__(BLOCK) void kt_flux_kernel(StageArgs A) {
+__global__ __launch_bounds__(BLOCK, stage_waves_per_simd<R>()) void kt_flux_kernel(StageArgs A) {
   __shared__ double acc_lds[15 * BLOCK];
   const CellPos c = cell_of_thread(A);
   if (!c.valid) return;

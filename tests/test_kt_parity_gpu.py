@@ -136,7 +136,7 @@ def test_field_roundtrip_and_errors():
         ctx.get(99)
 
 
-@pytest.mark.parametrize("variant,strip", [(0, 16), (1, 4), (2, 8), (0, 1), (2, 100), (3, 16), (3, 5)])
+@pytest.mark.parametrize("variant,strip", [(0, 16), (1, 4), (2, 8), (0, 1), (2, 100), (3, 16), (3, 5), (4, 16)])
 def test_kernel_arrangements_agree(monkeypatch, variant, strip):
     """Every stage-kernel arrangement and traversal strip gives the oracle's
     result (23x21x9, tabulated EoS, shear+bulk, 8 steps)."""
