@@ -314,6 +314,7 @@ extern "C" int gvh_destroy(gvh_context *c) {
   if (c->count) (void)hipFree(c->count);
   if (c->tab_pc) (void)hipFree(c->tab_pc);
   if (c->tab_tt) (void)hipFree(c->tab_tt);
+  if (c->validity) (void)hipFree(c->validity);
   fo_free(c->fo);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -517,6 +518,7 @@ StageArgs make_stage_args(gvh_context *c, int stage, double t) {
   StageArgs A{};
   stage_pointers(A, context_buffers(c), context_roles(c), stage);
   A.diag = c->diag;
+  A.reg = (stage == 2 && c->validity) ? c->validity + GVH_V_REGULATIONS * c->G : nullptr;
   A.G = c->G;
   A.nx = c->nx;
   A.ny = c->ny;

@@ -307,6 +307,7 @@ int admit(gvh_batch *b) {
     }
     BatchSlot S{};
     S.A = make_stage_args(c, 1, t0);
+    S.A.reg = nullptr;  // batched events keep no validity arrays
     S.B = context_buffers(c);
     S.R = context_roles(c);
     S.t0 = t0;

@@ -46,6 +46,7 @@ struct StageArgs {
   double *__restrict__ eout;
   double *__restrict__ pout;
   unsigned long long *__restrict__ diag;
+  double *__restrict__ reg;  // stage 2 stores the shear regulation factor here if set
   long long G;          // stride between variables
   int nx, ny, nz, ncx, ncy;
   double t;             // time of the Euler step

@@ -87,6 +87,7 @@ struct gvh_context {
   bool initialized = false;
   long long steps_done = 0;     // KT steps since the initial state was set
   gvh::FoState *fo = nullptr;   // freeze-out history and surface buffers
+  double *validity = nullptr;   // GVH_VALIDITY_FIELDS blocks of G doubles, on request
 
   int iQ() const { return 1 - iq; }
   int iqs() const { return 2; }

@@ -135,6 +135,32 @@ bool stale_view(const gvh_context *c, int flags) {
   return !(flags & GVH_FO_CURRENT_FIELDS) && (c->steps_done & 1);
 }
 
+// output() (FileIO.cpp:15-60): one ghosted field h as "x y z value" lines of
+// the interior cells into <outdir>/<name>_<t>.dat.
+int write_field_file(const gvh_context *c, const std::vector<double> &h, const char *name,
+                     double t, const std::string &outdir) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz, ncx = c->ncx, ncy = c->ncy;
+  const double dx = c->L.latticeSpacingX, dy = c->L.latticeSpacingY,
+               dz = c->L.latticeSpacingRapidity;
+  char fname[1024];
+  snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", outdir.c_str(), name, t);
+  File F;
+  F.f = fopen(fname, "w");
+  if (!F.f) return gvh_fail(GVH_EIO, "cannot write %s", fname);
+  for (int k = 2; k < nz + 2; ++k) {
+    const double z = (k - 2 - (nz - 1) / 2.) * dz;
+    for (int j = 2; j < ny + 2; ++j) {
+      const double y = (j - 2 - (ny - 1) / 2.) * dy;
+      for (int i = 2; i < nx + 2; ++i) {
+        const double x = (i - 2 - (nx - 1) / 2.) * dx;
+        fprintf(F.f, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z,
+                h[i + (size_t)ncx * (j + (size_t)ncy * k)]);
+      }
+    }
+  }
+  return GVH_OK;
+}
+
 // outputDynamicalQuantities (HydroPlugin.cpp:44-85) with output()
 // (FileIO.cpp:15-60): e, p, u, pi and Pi, one file per variable.
 int write_snapshot(gvh_context *c, double t, const std::string &outdir, int flags) {
@@ -158,9 +184,6 @@ int write_snapshot(gvh_context *c, double t, const std::string &outdir, int flag
                              {"pitn", nullptr, GVH_PITN}, {"pinn", nullptr, GVH_PINN}};
   if (c->bulk) items.push_back({"Pi", nullptr, GVH_PI});
   std::vector<double> h(G);
-  const int nx = c->nx, ny = c->ny, nz = c->nz, ncx = c->ncx, ncy = c->ncy;
-  const double dx = c->L.latticeSpacingX, dy = c->L.latticeSpacingY,
-               dz = c->L.latticeSpacingRapidity;
   for (const Item &it : items) {
     if (it.dev) {
       HIP_TRY(hipMemcpyAsync(h.data(), it.dev, G * sizeof(double), hipMemcpyDeviceToHost,
@@ -170,22 +193,31 @@ int write_snapshot(gvh_context *c, double t, const std::string &outdir, int flag
       int rc = gvh_copy_q_field(c, qd, it.field, h.data());
       if (rc) return rc;
     }
-    char fname[1024];
-    snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", outdir.c_str(), it.name, t);
-    File F;
-    F.f = fopen(fname, "w");
-    if (!F.f) return gvh_fail(GVH_EIO, "cannot write %s", fname);
-    for (int k = 2; k < nz + 2; ++k) {
-      const double z = (k - 2 - (nz - 1) / 2.) * dz;
-      for (int j = 2; j < ny + 2; ++j) {
-        const double y = (j - 2 - (ny - 1) / 2.) * dy;
-        for (int i = 2; i < nx + 2; ++i) {
-          const double x = (i - 2 - (nx - 1) / 2.) * dx;
-          fprintf(F.f, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z,
-                  h[i + (size_t)ncx * (j + (size_t)ncy * k)]);
-        }
-      }
-    }
+    int rc = write_field_file(c, h, it.name, t, outdir);
+    if (rc) return rc;
+  }
+  return GVH_OK;
+}
+
+// The validity fields under the names of the reference's (commented-out)
+// outputs, HydroPlugin.cpp:72-83, computed from the state at t as the
+// reference's last checkValidity call left them.
+int write_validity(gvh_context *c, double t, const std::string &outdir) {
+  static const struct {
+    const char *name;
+    int field;
+  } V[] = {{"regulations", GVH_V_REGULATIONS}, {"Rpi", GVH_V_RE_PI},
+           {"R2pi", GVH_V_RE2_PI},             {"RPi", GVH_V_RE_BULK},
+           {"R2Pi", GVH_V_RE2_BULK},           {"KnTaupi", GVH_V_KN_TAUPI},
+           {"KnTauPi", GVH_V_KN_TAUBULK},      {"taupi", GVH_V_TAUPI},
+           {"dxux", GVH_V_DXUX},               {"dyuy", GVH_V_DYUY},
+           {"theta", GVH_V_THETA}};
+  int rc = gvh_check_validity(c, t, 0);
+  if (rc) return rc;
+  std::vector<double> h(c->G);
+  for (const auto &v : V) {
+    if ((rc = gvh_get_validity(c, v.field, 1, h.data()))) return rc;
+    if ((rc = write_field_file(c, h, v.name, t, outdir))) return rc;
   }
   return GVH_OK;
 }
@@ -240,6 +272,7 @@ extern "C" void gvh_run_options_default(gvh_run_options *ro) {
   ro->snapshots = 1;
   ro->surface = 1;
   ro->verbose = 0;
+  ro->validity = 0;
 }
 
 extern "C" int gvh_run(const gvh_lattice_parameters *lattice, const gvh_ic_parameters *ic,
@@ -279,6 +312,8 @@ extern "C" int gvh_run(const gvh_lattice_parameters *lattice, const gvh_ic_param
                                S.uy.data(), S.un.data(), S.pi10.data(), S.Pi.data());
     if (rc) return rc;
   }
+  const bool validity = R.validity && R.snapshots;
+  if (validity && (rc = gvh_validity_enable(c))) return rc;
 
   // freeze-out finder (HydroPlugin.cpp:152-212)
   const bool fo_dims = (c->nx > 1 && c->ny > 1);
@@ -310,6 +345,7 @@ extern "C" int gvh_run(const gvh_lattice_parameters *lattice, const gvh_ic_param
                ec * HBARC, pc * HBARC);
       }
       if (R.snapshots && (rc = write_snapshot(c, t, output_dir, R.fo_flags))) return rc;
+      if (validity && (rc = write_validity(c, t, output_dir))) return rc;
     }
     if (surface) {
       const Clock::time_point a = Clock::now();

@@ -590,7 +590,9 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
                                 BULK ? acc[14 * BLOCK] : 0.0, e_s, A.diag);
 
   // regulateDissipativeCurrents (RegulateDissipativeCurrents.cu:16-134): only
-  // cells at or below e_reg are rescaled; the bulk factor is 1 unless p is 0
+  // cells at or below e_reg are rescaled; the bulk factor is 1 unless p is 0.
+  // Stage 2 records the shear factor when asked (:132, VALIDITY_DOMAIN).
+  double fs_rec = 1.0;
   if (!(pr.e > A.e_reg)) {
     const double t = A.tinf, t2 = t * t;
     const double pitt = acc[4 * BLOCK], pitx = acc[5 * BLOCK], pity = acc[6 * BLOCK],
@@ -612,6 +614,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
                        fmax(piu2 / den, piu3 / den)))));
     double fs = tanh(rho) / rho;
     if (fabs(rho) < 1.e-7) fs = 1.0;
+    fs_rec = fs;
 #pragma unroll 1
     for (int n = 4; n < 14; ++n) acc[n * BLOCK] *= fs;
     if constexpr (BULK) {
@@ -634,6 +637,10 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   if (j == A.ny + 1) { gh[ng++] = sy; gh[ng++] = 2 * sy; }
   if (k == NG) { gh[ng++] = -sz; gh[ng++] = -2 * sz; }
   if (k == A.nz + 1) { gh[ng++] = sz; gh[ng++] = 2 * sz; }
+
+  if constexpr (STAGE2) {
+    if (A.reg) A.reg[s] = fs_rec;
+  }
 
   auto put = [&](auto *base, auto val) {
     base[s] = val;

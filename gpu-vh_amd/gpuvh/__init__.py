@@ -31,6 +31,14 @@ FO_COLUMNS = 20
 FO_EXACT_TAU, FO_NORMAL_TAU, FO_CURRENT_FIELDS = 1, 2, 4
 FO_VARS = ["ux", "uy", "un", "e", "pixx", "pixy", "pixn", "piyy", "piyn", "Pi"]
 
+# validity diagnostics (include/gpuvh.h GVH_V_*): HydrodynamicValidity.cu
+(V_KN_TAUPI, V_KN_TAUBULK, V_RE_PI, V_RE2_PI, V_RE_BULK, V_RE2_BULK, V_TAUPI, V_DXUX, V_DYUY,
+ V_THETA, V_REGULATIONS) = range(11)
+VALIDITY_FIELDS = 11
+VALIDITY_CURRENT_PI = 1
+VALIDITY_NAMES = ["KnTaupi", "KnTauPi", "Rpi", "R2pi", "RPi", "R2Pi", "taupi", "dxux", "dyuy",
+                  "theta", "regulations"]
+
 FIELD_NAMES = ["ttt", "ttx", "tty", "ttn", "pitt", "pitx", "pity", "pitn", "pixx", "pixy",
                "pixn", "piyy", "piyn", "pinn", "Pi", "e", "p", "ut", "ux", "uy", "un",
                "utp", "uxp", "uyp", "unp"]
@@ -87,7 +95,8 @@ class Options(C.Structure):
 class RunOptions(C.Structure):
     """gvh_run_options: FREQ, FOFREQ (rhic/src/HydroPlugin.cpp:39-40), freeze-out flags."""
     _fields_ = [("freq", C.c_int), ("fofreq", C.c_int), ("fo_flags", C.c_int),
-                ("snapshots", C.c_int), ("surface", C.c_int), ("verbose", C.c_int)]
+                ("snapshots", C.c_int), ("surface", C.c_int), ("verbose", C.c_int),
+                ("validity", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -147,6 +156,9 @@ def lib():
         "gvh_synchronize": [C.c_void_p],
         "gvh_get_field": [C.c_void_p, C.c_int, C.c_int, dp],
         "gvh_count_above": [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)],
+        "gvh_validity_enable": [C.c_void_p],
+        "gvh_check_validity": [C.c_void_p, C.c_double, C.c_int],
+        "gvh_get_validity": [C.c_void_p, C.c_int, C.c_int, dp],
         "gvh_get_diagnostics": [C.c_void_p, C.POINTER(Diagnostics)],
         "gvh_energy_density_of_T": [C.c_void_p, C.c_double, dp],
         "gvh_equilibrium_pressure": [C.c_void_p, C.c_longlong, dp, dp],
@@ -354,6 +366,25 @@ class Context:
         _check(lib().gvh_get_field(self.h, field, int(ghosted), _ptr(out)))
         return out.reshape(shape, order="F")
 
+    def validity_enable(self):
+        """Allocate the validity arrays and record the regulation factor of
+        every later step (gvh_validity_enable)."""
+        _check(lib().gvh_validity_enable(self.h))
+
+    def check_validity(self, t, flags=0):
+        """checkValidity at proper time t (gvh_check_validity)."""
+        _check(lib().gvh_check_validity(self.h, t, flags))
+
+    def get_validity(self, field, ghosted=False):
+        """Validity field V_* as an (nx, ny, nz) array or the ghosted grid."""
+        if ghosted:
+            shape = (self.nx + 4, self.ny + 4, self.nz + 4)
+        else:
+            shape = (self.nx, self.ny, self.nz)
+        out = np.empty(int(np.prod(shape)), dtype=np.float64)
+        _check(lib().gvh_get_validity(self.h, field, int(ghosted), _ptr(out)))
+        return out.reshape(shape, order="F")
+
     def set_field(self, field, ghosted_array):
         a = self._flat(ghosted_array)
         _check(lib().gvh_set_field(self.h, field, _ptr(a)))
@@ -425,11 +456,13 @@ def cornelius_cells(dim, corners, dx, v0):
     return counts, cen, nrm
 
 
-def run_options(freq=10, fofreq=10, fo_flags=0, snapshots=True, surface=True, verbose=False):
+def run_options(freq=10, fofreq=10, fo_flags=0, snapshots=True, surface=True, verbose=False,
+                validity=False):
     ro = RunOptions()
     lib().gvh_run_options_default(C.byref(ro))
     ro.freq, ro.fofreq, ro.fo_flags = freq, fofreq, fo_flags
     ro.snapshots, ro.surface, ro.verbose = int(snapshots), int(surface), int(verbose)
+    ro.validity = int(validity)
     return ro
 
 

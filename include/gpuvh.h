@@ -227,6 +227,48 @@ int gvh_freezeout_slot(gvh_context *ctx, int logical, int var, double *host_out)
 int gvh_cornelius_cells(int dim, long long ncells, const double *corners, const double *dx,
                         double v0, int *counts, double *centroids, double *normals);
 
+/* ------------------------------------------------- validity diagnostics
+ * Knudsen and inverse Reynolds numbers per cell: checkValidityKernel
+ * (HydrodynamicValidity.cu:19-229) and the shear regulation factor that
+ * regulateDissipativeCurrents stores beside them
+ * (RegulateDissipativeCurrents.cu:132), i.e. the reference's VALIDITY_DOMAIN
+ * (DynamicalVariables.cuh:83-98; the two arrays it allocates but never
+ * writes are left out).  The reference fills them every step and never reads
+ * them back; here they are computed on request. */
+enum {
+  GVH_V_KN_TAUPI = 0, /* knudsenNumberTaupi  = 5 (eta/s) theta / T          */
+  GVH_V_KN_TAUBULK,   /* knudsenNumberTauPi  = theta / (1/tau_Pi)           */
+  GVH_V_RE_PI,        /* inverseReynoldsNumberPimunu = sqrt|pi:pi| / p      */
+  GVH_V_RE2_PI,       /* inverseReynoldsNumberTilde2Pimunu                  */
+  GVH_V_RE_BULK,      /* inverseReynoldsNumberPi = |Pi| / p                 */
+  GVH_V_RE2_BULK,     /* inverseReynoldsNumberTilde2Pi                      */
+  GVH_V_TAUPI,        /* taupi = 5 (eta/s) / T                              */
+  GVH_V_DXUX,         /* dxux, central difference                           */
+  GVH_V_DYUY,         /* dyuy                                               */
+  GVH_V_THETA,        /* theta, expansion rate                              */
+  GVH_V_REGULATIONS,  /* regulations: shear factor of the last step's end   */
+  GVH_VALIDITY_FIELDS
+};
+/* take pi and Pi from the current state instead of q^n of the last step */
+#define GVH_VALIDITY_CURRENT_PI 1
+
+/* Allocate the validity arrays (GVH_VALIDITY_FIELDS blocks of the ghosted
+ * grid; regulations start at 1 as DynamicalVariables.cu:81) and record the
+ * regulation factor in every later step. */
+int gvh_validity_enable(gvh_context *ctx);
+/* checkValidity (HydrodynamicValidity.cu:231-235) at proper time t from the
+ * context's state, as the reference calls it at the end of a step
+ * (FullyDiscreteKurganovTadmorScheme.cu:137) and before the first
+ * (HydroPlugin.cpp:222): e, p, u and u_prev of the current state; pi and Pi
+ * of q^n, the conserved state before the last step (the reference passes
+ * d_q, which its run loop swaps only after the step), unless flags has
+ * GVH_VALIDITY_CURRENT_PI or no step has been taken.  Enables the arrays on
+ * first use.  Asynchronous on the context's stream. */
+int gvh_check_validity(gvh_context *ctx, double t, int flags);
+/* Copy validity field f: interior cells (ghosted == 0, nx*ny*nz values, x
+ * fastest) or the full ghosted grid. */
+int gvh_get_validity(gvh_context *ctx, int field, int ghosted, double *host_out);
+
 /* -------------------------------------------------------------- run loop */
 typedef struct gvh_run_options {
   int freq;      /* snapshot cadence FREQ (HydroPlugin.cpp:39); reference 10   */
@@ -235,6 +277,8 @@ typedef struct gvh_run_options {
   int snapshots; /* write the FREQ snapshots (outputDynamicalQuantities, :44) */
   int surface;   /* write <root_dir>/output/surface.dat                        */
   int verbose;   /* progress lines on stdout                                   */
+  int validity;  /* with snapshots, also write the validity fields (the
+                    reference's commented-out outputs, HydroPlugin.cpp:72-83)  */
 } gvh_run_options;
 
 typedef struct gvh_run_stats {

@@ -818,6 +818,175 @@ static void regulate(kto_ctx *c, double t, double **q, double **u) {
       }
 }
 
+/* ------------------------------------------------ validity diagnostics */
+
+/* checkValidityKernel, HydrodynamicValidity.cu:29-227, for one cell: e, p,
+ * u, u_prev of the cell, du[d][m] = d_d u^m the central differences of
+ * :50-68 (d = x, y, eta), pi[10] = tt tx ty tn xx xy xn yy yn nn, Pi the
+ * bulk pressure (0 without bulk).  The vorticity is the older convention
+ * this kernel keeps (:116-121), not the one of the source terms. */
+void kto_validity_cell(const kto_params *P, double t, double e_s, double p_s,
+                       const double *u, const double *up, const double du[3][4],
+                       const double *pi, double Pi, double *out) {
+  const double etabar = P->etabar;
+  double ut = u[0], ux = u[1], uy = u[2], un = u[3];
+  double utp = up[0], uxp = up[1], uyp = up[2], unp = up[3];
+  double dxut = du[0][0], dxux = du[0][1], dxuy = du[0][2], dxun = du[0][3];
+  double dyut = du[1][0], dyux = du[1][1], dyuy = du[1][2], dyun = du[1][3];
+  double dnut = du[2][0], dnux = du[2][1], dnuy = du[2][2], dnun = du[2][3];
+  double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
+
+  /* :76-83 */
+  double T = eos_T(P, e_s);
+  double cs2 = eos_cs2(P, e_s);
+  double a = F_THIRD - cs2;
+  double a2 = a * a;
+  double lambda_Pipi = F_1P6 * a;
+  double zetabar = kto_zeta_over_s(P, T);
+  double tauPiInv = 15 * a2 * (T / zetabar);
+
+  /* :86-100 */
+  double dtut = (ut - utp) / P->dt, dtux = (ux - uxp) / P->dt;
+  double dtuy = (uy - uyp) / P->dt, dtun = (un - unp) / P->dt;
+  double Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
+  double dut = Dut - t * un * un;
+  double dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
+  double duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
+  double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
+  double Dun = -t2 * dun - 2 * t * ut * un;
+  double theta = ut / t + dtut + dxux + dyuy + dnun;
+
+  /* shear tensor sigma^{mu nu}, :103-113 */
+  double theta3 = theta / 3;
+  double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1) * theta3;
+  double stx = -(t * un2 * ux) / 2 + (dtux - dxut) / 2 - (ux * dut + ut * dux) / 2 + ut * ux * theta3;
+  double sty = -(t * un2 * uy) / 2 + (dtuy - dyut) / 2 - (uy * dut + ut * duy) / 2 + ut * uy * theta3;
+  double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (dtun - dnut / t2) / 2 -
+               (un * dut + ut * dun) / 2 + ut * un * theta3;
+  double sxx = -(dxux + ux * dux) + (1 + ux * ux) * theta3;
+  double sxy = -(dxuy + dyux) / 2 - (uy * dux + ux * duy) / 2 + ux * uy * theta3;
+  double sxn = -ut * ux * un / t - (dxun + dnux / t2) / 2 - (un * dux + ux * dun) / 2 + ux * un * theta3;
+  double syy = -(dyuy + uy * duy) + (1 + uy * uy) * theta3;
+  double syn = -ut * uy * un / t - (dyun + dnuy / t2) / 2 - (un * duy + uy * dun) / 2 + uy * un * theta3;
+  double snn = -ut * (1 + 2 * t2 * un2) / t3 - dnun / t2 - un * dun + (1 / t2 + un2) * theta3;
+
+  /* vorticity, :116-128 */
+  double wtx = (dtux + dxut) / 2 + (ux * dut - ut * dux) / 2 + t * un2 * ux / 2;
+  double wty = (dtuy + dyut) / 2 + (uy * dut - ut * duy) / 2 + t * un2 * uy / 2;
+  double wtn = (t2 * dtun + 2 * t * un + dnut) / 2 + (t2 * un * dut - ut * Dun) + t3 * un * un2 / 2;
+  double wxy = (dyux - dxuy) / 2 + (uy * dux - ux * duy) / 2;
+  double wxn = (dnux - t2 * dxun) / 2 + (t2 * un * dux - ux * Dun) / 2;
+  double wyn = (dnuy - t2 * dyun) / 2 + (t2 * un * duy - uy * Dun) / 2;
+  double wxt = wtx, wyt = wty, wnt = wtn / t2, wyx = -wxy, wnx = -wxn / t2, wny = -wyn / t2;
+
+  double pitt = pi[0], pitx = pi[1], pity = pi[2], pitn = pi[3], pixx = pi[4];
+  double pixy = pi[5], pixn = pi[6], piyy = pi[7], piyn = pi[8], pinn = pi[9];
+
+  /* I2, I3, I4, :148-192 */
+  double I2tt = theta * pitt, I2tx = theta * pitx, I2ty = theta * pity, I2tn = theta * pitn;
+  double I2xx = theta * pixx, I2xy = theta * pixy, I2xn = theta * pixn, I2yy = theta * piyy;
+  double I2yn = theta * piyn, I2nn = theta * pinn;
+  double I3tt = 2 * (pitx * wtx + pity * wty + pitn * wtn);
+  double I3tx = pitt * wxt + pity * wxy + pitn * wxn + pixx * wtx + pixy * wty + pixn * wtn;
+  double I3ty = pitt * wyt + pitx * wyx + pitn * wyn + pixy * wtx + piyy * wty + piyn * wtn;
+  double I3tn = pitt * wnt + pitx * wnx + pity * wny + pixn * wtx + piyn * wty + pinn * wtn;
+  double I3xx = 2 * (pitx * wxt + pixy * wxy + pixn * wxn);
+  double I3xy = pitx * wyt + pity * wxt + pixx * wyx + piyy * wxy + pixn * wyn + piyn * wxn;
+  double I3xn = pitx * wnt + pitn * wxt + pixx * wnx + pixy * wny + piyn * wxy + pinn * wxn;
+  double I3yy = 2 * (pity * wyt + pixy * wyx + piyn * wyn);
+  double I3yn = pity * wnt + pitn * wyt + pixy * wnx + pixn * wyx + piyy * wny + pinn * wyn;
+  double I3nn = 2 * (pitn * wnt + pixn * wnx + piyn * wny);
+  double ux2 = ux * ux, uy2 = uy * uy;
+  double ps = pitt * stt - 2 * pitx * stx - 2 * pity * sty + pixx * sxx + 2 * pixy * sxy +
+              piyy * syy - 2 * pitn * stn * t2 + 2 * pixn * sxn * t2 + 2 * piyn * syn * t2 +
+              pinn * snn * t2 * t2;
+  double ps3 = ps / 3;
+  double I4tt = (pitt * stt - pitx * stx - pity * sty - t2 * pitn * stn) - (1 - ut2) * ps3;
+  double I4tx = (pitt * stx + pitx * stt) / 2 - (pitx * sxx + pixx * stx) / 2 -
+                (pity * sxy + pixy * sty) / 2 - t2 * (pitn * sxn + pixn * stn) / 2 + (ut * ux) * ps3;
+  double I4ty = (pitt * sty + pity * stt) / 2 - (pitx * sxy + pixy * stx) / 2 -
+                (pity * syy + piyy * sty) / 2 - t2 * (pitn * syn + piyn * stn) / 2 + (ut * uy) * ps3;
+  double I4tn = (pitt * stn + pitn * stt) / 2 - (pitx * sxn + pixn * stx) / 2 -
+                (pity * syn + piyn * sty) / 2 - t2 * (pitn * snn + pinn * stn) / 2 + (ut * un) * ps3;
+  double I4xx = (pitx * stx - pixx * sxx - pixy * sxy - t2 * pixn * sxn) + (1 + ux2) * ps3;
+  double I4xy = (pitx * sty + pity * stx) / 2 - (pixx * sxy + pixy * sxx) / 2 -
+                (pixy * syy + piyy * sxy) / 2 - t2 * (pixn * syn + piyn * sxn) / 2 + (ux * uy) * ps3;
+  double I4xn = (pitx * stn + pitn * stx) / 2 - (pixx * sxn + pixn * sxx) / 2 -
+                (pixy * syn + piyn * sxy) / 2 - t2 * (pixn * snn + pinn * sxn) / 2 + (ux * un) * ps3;
+  double I4yy = (pity * sty - pixy * sxy - piyy * syy - t2 * piyn * syn) + (1 + uy2) * ps3;
+  double I4yn = (pity * stn + pitn * sty) / 2 - (pixy * sxn + pixn * sxy) / 2 -
+                (piyy * syn + piyn * syy) / 2 - t2 * (piyn * snn + pinn * syn) / 2 + (uy * un) * ps3;
+  double I4nn = (pitn * stn - pixn * sxn - piyn * syn - t2 * pinn * snn) + (1 / t2 + un2) * ps3;
+
+  /* :194-212 */
+  double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy +
+                piyy * piyy - 2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 +
+                pinn * pinn * t2 * t2;
+  double ss = stt * stt - 2 * stx * stx - 2 * sty * sty + sxx * sxx + 2 * sxy * sxy + syy * syy -
+              2 * stn * stn * t2 + 2 * sxn * sxn * t2 + 2 * syn * syn * t2 + snn * snn * t2 * t2;
+  double Jtt = delta_pipi * I2tt - I3tt + tau_pipi * I4tt - lambda_piPi * Pi * stt;
+  double Jtx = delta_pipi * I2tx - I3tx + tau_pipi * I4tx - lambda_piPi * Pi * stx;
+  double Jty = delta_pipi * I2ty - I3ty + tau_pipi * I4ty - lambda_piPi * Pi * sty;
+  double Jtn = delta_pipi * I2tn - I3tn + tau_pipi * I4tn - lambda_piPi * Pi * stn;
+  double Jxx = delta_pipi * I2xx - I3xx + tau_pipi * I4xx - lambda_piPi * Pi * sxx;
+  double Jxy = delta_pipi * I2xy - I3xy + tau_pipi * I4xy - lambda_piPi * Pi * sxy;
+  double Jxn = delta_pipi * I2xn - I3xn + tau_pipi * I4xn - lambda_piPi * Pi * sxn;
+  double Jyy = delta_pipi * I2yy - I3yy + tau_pipi * I4yy - lambda_piPi * Pi * syy;
+  double Jyn = delta_pipi * I2yn - I3yn + tau_pipi * I4yn - lambda_piPi * Pi * syn;
+  double Jnn = delta_pipi * I2nn - I3nn + tau_pipi * I4nn - lambda_piPi * Pi * snn;
+  double J = -delta_PiPi * Pi * theta + lambda_Pipi * ps;
+  double JJ = Jtt * Jtt - 2 * Jtx * Jtx - 2 * Jty * Jty + Jxx * Jxx + 2 * Jxy * Jxy + Jyy * Jyy -
+              2 * Jtn * Jtn * t2 + 2 * Jxn * Jxn * t2 + 2 * Jyn * Jyn * t2 + Jnn * Jnn * t2 * t2;
+
+  /* :214-227 */
+  out[KTO_V_RE_PI] = sqrt(fabs(pipi)) / p_s;
+  out[KTO_V_RE2_PI] = T / 2 / etabar / (e_s + p_s) * sqrt(fabs(JJ / ss));
+  out[KTO_V_RE_BULK] = fabs(Pi) / p_s;
+  out[KTO_V_RE2_BULK] = fabs(J / zetabar / theta) * T / (e_s + p_s);
+  out[KTO_V_KN_TAUPI] = 5 * etabar * theta / T;
+  out[KTO_V_KN_TAUBULK] = theta / tauPiInv;
+  out[KTO_V_TAUPI] = 5 * etabar / T;
+  out[KTO_V_DXUX] = dxux;
+  out[KTO_V_DYUY] = dyuy;
+  out[KTO_V_THETA] = theta;
+}
+
+/* checkValidity (HydrodynamicValidity.cu:231-235) over the interior.  The
+ * reference calls it at the end of twoStepRungeKutta with d_q, which the run
+ * loop swaps only afterwards (FullyDiscreteKurganovTadmorScheme.cu:137,
+ * HydroPlugin.cpp:422): pi and Pi are then those of q^n, the buffer that is
+ * Q after kto_step.  prev_q selects that buffer; prev_q == 0 reads the
+ * current q (the call before the first step, HydroPlugin.cpp:222).  out
+ * holds KTO_V_N ghosted arrays; interior cells are written, the last field
+ * with the regulation factors of the last regulation. */
+void kto_validity(void *h, double t, int prev_q, double *out) {
+  kto_ctx *c = (kto_ctx *)h;
+  const kto_params *P = &c->P;
+  double **q = prev_q ? c->Q : c->q;
+  const double fx = 1 / P->dx / 2, fy = 1 / P->dy / 2, fz = 1 / P->dz / 2;
+  const long sy = c->ncx, sz = (long)c->ncx * c->ncy;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = NG; k < P->nz + NG; ++k)
+    for (int j = NG; j < P->ny + NG; ++j)
+      for (int i = NG; i < P->nx + NG; ++i) {
+        long s = idx(c, i, j, k);
+        double uc[4], upc[4], du[3][4], pi[10], v[KTO_V_N];
+        for (int m = 0; m < 4; ++m) {
+          const double *um = c->u[m];
+          uc[m] = um[s];
+          upc[m] = c->up[m][s];
+          du[0][m] = (um[s + 1] - um[s - 1]) * fx;
+          du[1][m] = (um[s + sy] - um[s - sy]) * fy;
+          du[2][m] = (um[s + sz] - um[s - sz]) * fz;
+        }
+        for (int m = 0; m < 10; ++m) pi[m] = q[4 + m][s];
+        double Pi = P->bulk ? q[14][s] : 0.0;
+        kto_validity_cell(P, t, c->e[s], c->p[s], uc, upc, du, pi, Pi, v);
+        for (int f = 0; f < KTO_V_REG; ++f) out[f * c->G + s] = v[f];
+        out[KTO_V_REG * c->G + s] = c->reg[s];
+      }
+}
+
 /* setGhostCells + kernels I/J/K, GhostCells.cu:15-144: two layers per face
  * copy the nearest interior plane; edges and corners are left untouched. */
 static void ghost_copy(kto_ctx *c, long s, long sBC, double **q, double **u) {
@@ -923,7 +1092,8 @@ void *kto_create(const kto_params *P) {
   }
   c->e = (double *)calloc(c->G, sizeof(double));
   c->p = (double *)calloc(c->G, sizeof(double));
-  c->reg = (double *)calloc(c->G, sizeof(double));
+  c->reg = (double *)malloc(sizeof(double) * c->G);
+  for (long s = 0; s < c->G; ++s) c->reg[s] = 1.0; /* DynamicalVariables.cu:81 */
   c->q = c->qa;
   c->Q = c->qb;
   c->qS = c->qs;

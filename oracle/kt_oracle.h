@@ -96,6 +96,26 @@ void kto_source_cell(const kto_params *P, double t, const double *Q, double e,
 /* shear regulation factor of one cell (1 when e > e_reg) */
 double kto_regulate_factor(double t, const double *pi, double e, double p,
                            const double *u, double e_reg);
+/* validity diagnostics (HydrodynamicValidity.cu), output field order */
+enum {
+  KTO_V_KN_TAUPI = 0, /* knudsenNumberTaupi            */
+  KTO_V_KN_TAUBULK,   /* knudsenNumberTauPi            */
+  KTO_V_RE_PI,        /* inverseReynoldsNumberPimunu   */
+  KTO_V_RE2_PI,       /* inverseReynoldsNumberTilde2Pimunu */
+  KTO_V_RE_BULK,      /* inverseReynoldsNumberPi       */
+  KTO_V_RE2_BULK,     /* inverseReynoldsNumberTilde2Pi */
+  KTO_V_TAUPI, KTO_V_DXUX, KTO_V_DYUY, KTO_V_THETA,
+  KTO_V_REG,          /* regulations (RegulateDissipativeCurrents.cu:132) */
+  KTO_V_N
+};
+/* checkValidityKernel of one cell: du[d][m] = d_d u^m central differences,
+ * pi[10] tt..nn, out[KTO_V_REG] (all but the regulation factor). */
+void kto_validity_cell(const kto_params *P, double t, double e, double p,
+                       const double *u, const double *up, const double du[3][4],
+                       const double *pi, double Pi, double *out);
+/* The validity arrays of the lattice (KTO_V_N ghosted arrays, interior
+ * written); prev_q != 0 takes pi, Pi from q^n of the last step. */
+void kto_validity(void *h, double t, int prev_q, double *out);
 /* Newton iteration count of the last kto_inferred call (non-conformal). */
 int kto_last_newton_iters(void);
 

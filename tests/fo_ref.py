@@ -152,10 +152,13 @@ def host_view(cur, prev, steps_done, flags):
 
 
 def run_loop(orc, geom, nt, e_fo, eos_T_p, bulk, fofreq=10, freq=10, flags=0, surface=True,
-             snapshot_fields=()):
+             snapshot_fields=(), validity=False):
     """The run loop over an oracle whose initial state is set.  Returns
     (rows, steps, frozen, snapshots) with snapshots {t: {field id: array}}
-    of the listed oracle field ids (u and pi as the host copy sees them)."""
+    of the listed oracle field ids (u and pi as the host copy sees them);
+    with validity, snapshots[t]["validity"] holds the validity arrays as the
+    reference's last checkValidity left them (HydroPlugin.cpp:222,
+    FullyDiscreteKurganovTadmorScheme.cu:137)."""
     t0, dt = geom["t0"], geom["dt"]
     ring = Ring(fofreq)
     rows = []
@@ -174,6 +177,8 @@ def run_loop(orc, geom, nt, e_fo, eos_T_p, bulk, fofreq=10, freq=10, flags=0, su
             for f in snapshot_fields:
                 is_u_pi = f in (O.UT, O.UX, O.UY, O.UN) or O.PITT <= f <= O.PI
                 snaps[t][f] = prev_full[f] if (stale and is_u_pi) else orc.get(f)
+            if validity:
+                snaps[t]["validity"] = orc.validity(t, prev_q=steps > 0)
         if surface:
             ring.record(n, host_view(cur, prev, steps, flags))
             if n % fofreq == fofreq - 1:

@@ -20,6 +20,10 @@ PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, PIYN, PINN, PI = range(4, 15)
 E, P, UT, UX, UY, UN = 15, 16, 17, 18, 19, 20
 UTP, UXP, UYP, UNP = 21, 22, 23, 24
 REG = 25
+# validity diagnostics (HydrodynamicValidity.cu), oracle field order
+(V_KN_TAUPI, V_KN_TAUBULK, V_RE_PI, V_RE2_PI, V_RE_BULK, V_RE2_BULK, V_TAUPI, V_DXUX, V_DYUY,
+ V_THETA, V_REG) = range(11)
+V_N = 11
 
 
 class Params(C.Structure):
@@ -76,6 +80,9 @@ def lib():
         L.kto_regulate_factor.argtypes = [C.c_double, dp, C.c_double, C.c_double, dp, C.c_double]
         L.kto_source_cell.argtypes = [C.POINTER(Params), C.c_double, dp, C.c_double,
                                       C.c_double, dp, dp, dp, dp, dp]
+        L.kto_validity_cell.argtypes = [C.POINTER(Params), C.c_double, C.c_double, C.c_double,
+                                        dp, dp, dp, dp, C.c_double, dp]
+        L.kto_validity.argtypes = [C.c_void_p, C.c_double, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -160,6 +167,15 @@ class Oracle:
         lib().kto_get(self.h, field, 0, int(from_stage), _ptr(out))
         return out.reshape((self.nx, self.ny, self.nz), order="F")
 
+    def validity(self, t, prev_q):
+        """The V_N validity arrays as (V_N, nx, ny, nz) (interior cells);
+        prev_q: pi and Pi from q^n of the last step, as the reference."""
+        n = lib().kto_ghosted_size(self.h)
+        out = np.zeros(V_N * n, dtype=np.float64)
+        lib().kto_validity(self.h, t, int(prev_q), _ptr(out))
+        g = out.reshape((V_N, self.nz + 4, self.ny + 4, self.nx + 4))
+        return np.ascontiguousarray(g[:, 2:-2, 2:-2, 2:-2].transpose(0, 3, 2, 1))
+
     def set_field(self, field, ghosted_array):
         a = np.ascontiguousarray(np.asarray(ghosted_array, dtype=np.float64).ravel(order="F"))
         lib().kto_set_field(self.h, field, _ptr(a))
@@ -192,6 +208,16 @@ def source_cell(params, t, Q, e, p, u, up, du, dp):
     lib().kto_source_cell(C.byref(params), t, _ptr(arr[0]), e, p, _ptr(arr[1]), _ptr(arr[2]),
                           _ptr(arr[3]), _ptr(arr[4]), _ptr(S))
     return S
+
+
+def validity_cell(params, t, e, p, u, up, du, pi10, Pi):
+    """checkValidityKernel of one cell; du is (3, 4) = d_i u^m.  Returns the
+    first V_REG validity values."""
+    arr = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (u, up, du, pi10)]
+    out = np.zeros(V_N, dtype=np.float64)
+    lib().kto_validity_cell(C.byref(params), t, e, p, _ptr(arr[0]), _ptr(arr[1]), _ptr(arr[2]),
+                            _ptr(arr[3]), Pi, _ptr(out))
+    return out[:V_REG]
 
 
 def regulate_factor(t, pi10, e, p, u, e_reg):
