@@ -182,7 +182,9 @@ int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T
   T = EosTable{};
   if (H.kind != EOS_TABLE) return GVH_OK;
   const int N = H.n;
-  std::vector<double> pc(4 * (size_t)N), tt(2 * (size_t)N);
+  // one allocation: 4N doubles, then the same 4N values as floats
+  std::vector<double> pc(4 * (size_t)N + 2 * (size_t)N), tt(2 * (size_t)N);
+  float *pcf = reinterpret_cast<float *>(pc.data() + 4 * (size_t)N);
   for (int k = 0; k < N; ++k) {
     const int k1 = k + 1 < N ? k + 1 : k;
     pc[4 * k + 0] = H.p[k];
@@ -191,6 +193,7 @@ int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T
     pc[4 * k + 3] = H.cs2[k1] - H.cs2[k];
     tt[2 * k + 0] = H.T[k];
     tt[2 * k + 1] = H.T[k1] - H.T[k];
+    for (int m = 0; m < 4; ++m) pcf[4 * k + m] = (float)pc[4 * k + m];
   }
   HIP_TRY(hipMalloc(dev_pc, pc.size() * sizeof(double)));
   HIP_TRY(hipMalloc(dev_tt, tt.size() * sizeof(double)));
@@ -198,6 +201,7 @@ int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T
   HIP_TRY(hipMemcpyAsync(*dev_tt, tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   T.pc = (const double4 *)*dev_pc;
+  T.pcf = (const float4 *)((const double *)*dev_pc + 4 * (size_t)N);
   T.tt = (const double2 *)*dev_tt;
   T.n = N;
   T.M = H.M;
@@ -208,6 +212,11 @@ int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T
   T.e_last = node_e(H, N - 1);
   T.inv_first_span = 1.0 / (T.e_second - T.e_first);
   T.inv_last_span = 1.0 / (T.e_last - T.e_pen);
+  T.f_first = (float)T.e_first;
+  T.f_pen = (float)T.e_pen;
+  T.f_last = (float)T.e_last;
+  T.f_inv_first_span = (float)T.inv_first_span;
+  T.f_inv_last_span = (float)T.inv_last_span;
   return GVH_OK;
 }
 

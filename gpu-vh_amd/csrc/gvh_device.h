@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gvh {
 
 constexpr int NG = 2;             // ghost layers (LatticeParameters.h:13-15)
@@ -27,11 +29,13 @@ enum { DIAG_NOT_CONVERGED = 0, DIAG_E_NAN = 1, DIAG_E_CLAMPED = 2, DIAG_N = 4 };
 struct EosTable {
   const double4 *pc;  // {p_k, p_{k+1}-p_k, cs2_k, cs2_{k+1}-cs2_k}
   const double2 *tt;  // {T_k, T_{k+1}-T_k}
+  const float4 *pcf;  // pc rounded to FP32 (the mixed form's flux root-finds)
   int n, M, emin_exp;
   double e_first, e_second;  // e_0, e_1
   double e_pen, e_last;      // e_{n-2}, e_{n-1}
   double inv_first_span;     // 1/(e_1 - e_0)
   double inv_last_span;      // 1/(e_{n-1} - e_{n-2})
+  float f_first, f_pen, f_last, f_inv_first_span, f_inv_last_span;  // the same in FP32
 };
 
 struct StageArgs {
@@ -178,6 +182,15 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return x == 0.0 ? x : fma(d, h, g);
 }
 
+// The same operations in the arithmetic type of a root-find: FP64 as above,
+// FP32 from the hardware approximations (about 1 ulp).
+__device__ __forceinline__ double rcp_t(double b) { return rcp_nr(b); }
+__device__ __forceinline__ double div_t(double a, double b) { return div_nr(a, b); }
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt_nr(x); }
+__device__ __forceinline__ float rcp_t(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float div_t(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float sqrt_t(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // ---------------------------------------------------------------- EoS
 struct TabPos {
   int k;
@@ -272,6 +285,61 @@ __device__ __forceinline__ double eos_p(const EosTable &T, double e) {
   }
 }
 
+// p and c_s^2 of the tabulated EoS in FP32 (tab_locate with float
+// arithmetic, FP32 node values).
+__device__ __forceinline__ void tab_p_cs2_f(const EosTable &T, float e, float &p, float &cs2) {
+  int k;
+  float fr;
+  if (e >= T.f_first && e < T.f_last) {
+    int E;
+    const float m = frexpf(e, &E);
+    const float f = (2.0f * m - 1.0f) * (float)T.M;
+    const int j = (int)f;
+    fr = f - (float)j;
+    k = (E - 1 - T.emin_exp) * T.M + j;
+    if (k > T.n - 2) {
+      k = T.n - 2;
+      fr += 1.0f;
+    }
+  } else if (e >= T.f_last) {
+    k = T.n - 2;
+    fr = (e - T.f_pen) * T.f_inv_last_span;
+  } else {
+    k = 0;
+    fr = (e - T.f_first) * T.f_inv_first_span;
+  }
+  const float4 v = T.pcf[k];
+  p = v.x + fr * v.y;
+  cs2 = v.z + fr * v.w;
+}
+
+// eos_p_cs2 / eos_p in the arithmetic type F (float only for the conformal
+// and tabulated EoS).
+template <int EOS, typename F>
+__device__ __forceinline__ void eos_p_cs2_t(const EosTable &T, F e, F &p, F &cs2) {
+  if constexpr (std::is_same<F, double>::value) {
+    eos_p_cs2<EOS>(T, e, p, cs2);
+  } else {
+    static_assert(EOS != EOS_WB, "the WB fits stay in FP64");
+    if constexpr (EOS == EOS_CONFORMAL) {
+      p = e / 3.0f;
+      cs2 = 0.0f;
+    } else {
+      tab_p_cs2_f(T, e, p, cs2);
+    }
+  }
+}
+template <int EOS, typename F>
+__device__ __forceinline__ F eos_p_t(const EosTable &T, F e) {
+  if constexpr (std::is_same<F, double>::value) {
+    return eos_p<EOS>(T, e);
+  } else {
+    F p, c;
+    eos_p_cs2_t<EOS, F>(T, e, p, c);
+    return p;
+  }
+}
+
 // effectiveTemperature, EquationOfState.cu:119-148
 template <int EOS>
 __device__ __forceinline__ double eos_T(const EosTable &T, double e) {
@@ -315,24 +383,23 @@ __device__ __forceinline__ double zeta_over_s(double T, double tpeak) {
 // e0 - f/fp with fp = 1 - num/den is evaluated as e0 - f den/(den - num),
 // and 1/e0 is shared: three divisions per iteration instead of five, equal
 // to the reference's expression up to rounding.
-template <int EOS, bool DAMPED>
-__device__ __forceinline__ bool newton_step(const EosTable &T, double &e0, double M0, double M,
-                                            double Pi) {
-  double p, cs2;
-  eos_p_cs2<EOS>(T, e0, p, cs2);
-  const double ie0 = rcp_nr(e0);
-  const double cst2 = p * ie0;
-  const double A = fma(M0, 1.0 - cst2, Pi);
-  const double B = fma(M0, M0 + Pi, -M);
-  const double H = sqrt_nr(fabs(A * A + 4.0 * cst2 * B));
-  const double D = div_nr(A - H, 2.0 * cst2);
-  const double f = e0 + D;
-  const double dc = cs2 - cst2;
-  const double num = dc * (B + D * H - dc * cst2 * D * M0 * ie0);
-  const double den = cst2 * e0 * H;
-  const double step = div_nr((DAMPED ? 0.9 * f : f) * den, den - num);
-  const double e = e0 - step;
-  const bool done = fabs(e - e0) <= 0.001 * fabs(e);
+template <int EOS, bool DAMPED, typename F = double>
+__device__ __forceinline__ bool newton_step(const EosTable &T, F &e0, F M0, F M, F Pi) {
+  F p, cs2;
+  eos_p_cs2_t<EOS, F>(T, e0, p, cs2);
+  const F ie0 = rcp_t(e0);
+  const F cst2 = p * ie0;
+  const F A = fma(M0, F(1) - cst2, Pi);
+  const F B = fma(M0, M0 + Pi, -M);
+  const F H = sqrt_t(fabs(A * A + F(4) * cst2 * B));
+  const F D = div_t(A - H, F(2) * cst2);
+  const F f = e0 + D;
+  const F dc = cs2 - cst2;
+  const F num = dc * (B + D * H - dc * cst2 * D * M0 * ie0);
+  const F den = cst2 * e0 * H;
+  const F step = div_t((DAMPED ? F(0.9) * f : f) * den, den - num);
+  const F e = e0 - step;
+  const bool done = fabs(e - e0) <= F(0.001) * fabs(e);
   e0 = e;
   return done;
 }
@@ -368,13 +435,16 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
 // that chain.  A wave already iterates until its slowest lane converges, so
 // the instruction count matches N sequential solves while the N table
 // gathers and dependency chains overlap.
-template <int EOS, int N>
-__device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev, const double *M0,
-                                                 const double *M, const double *Pi, double *e,
+template <int EOS, int N, typename F = double>
+__device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, const F *M0,
+                                                 const F *M, const F *Pi, F *e,
                                                  unsigned long long *diag) {
   if constexpr (EOS == EOS_CONFORMAL) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) e[k] = fabs(sqrt(fabs(4.0 * M0[k] * M0[k] - 3.0 * M[k])) - M0[k]);
+    for (int k = 0; k < N; ++k) {
+      const F r = fabs(F(4) * M0[k] * M0[k] - F(3) * M[k]);
+      e[k] = fabs((std::is_same<F, double>::value ? sqrt(r) : sqrt_t(r)) - M0[k]);
+    }
   } else {
     unsigned act = (1u << N) - 1u;
 #pragma unroll
@@ -383,8 +453,8 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev
     for (int it = 0; it < MAX_ITERS / 2 && act; ++it) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        double ek = e[k];
-        const bool done = newton_step<EOS, false>(T, ek, M0[k], M[k], Pi[k]);
+        F ek = e[k];
+        const bool done = newton_step<EOS, false, F>(T, ek, M0[k], M[k], Pi[k]);
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
@@ -397,8 +467,8 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, double ePrev
     for (int it = MAX_ITERS / 2; it < MAX_ITERS && act; ++it) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        double ek = e[k];
-        const bool done = newton_step<EOS, true>(T, ek, M0[k], M[k], Pi[k]);
+        F ek = e[k];
+        const bool done = newton_step<EOS, true, F>(T, ek, M0[k], M[k], Pi[k]);
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
@@ -476,38 +546,48 @@ __device__ __forceinline__ double flux_velocity(const EosTable &T, double t, dou
 // flux_velocity of N half-site states of one direction that share the
 // guess ePrev, with their root-finds in lockstep (solve_e_lockstep).
 // W[k] = (T^tt, T^tx, T^ty, T^tn, pi^tt, pi^tx, pi^ty, pi^tn, Pi).
+// In the mixed form (R = float) the root-finds of the flux states run in FP32
+// for the conformal and tabulated EoS: their results only set the flux
+// velocities of FP32 fluxes.  The WB fits (powers up to e^13) overflow FP32
+// and stay FP64, as does every cell's own inversion.
+template <int EOS, typename R>
+using flux_root_t =
+    typename std::conditional<(sizeof(R) == 4 && EOS != EOS_WB), float, double>::type;
+
 template <int EOS, int DIR, int N, typename R>
 __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double t,
                                                        const R (*W)[9], double ePrev,
                                                        unsigned long long *diag, double *v) {
-  double M0[N], M[N], Md[N], Pi[N], e[N];
+  using F = flux_root_t<EOS, R>;
+  F M0[N], M[N], Md[N], Pi[N], e[N];
+  const F tf = F(t);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    M0[k] = (double)W[k][0] - (double)W[k][4];
-    const double M1 = (double)W[k][1] - (double)W[k][5], M2 = (double)W[k][2] - (double)W[k][6],
-                 M3 = (double)W[k][3] - (double)W[k][7];
-    M[k] = M1 * M1 + M2 * M2 + t * t * M3 * M3;
+    M0[k] = (F)W[k][0] - (F)W[k][4];
+    const F M1 = (F)W[k][1] - (F)W[k][5], M2 = (F)W[k][2] - (F)W[k][6],
+            M3 = (F)W[k][3] - (F)W[k][7];
+    M[k] = M1 * M1 + M2 * M2 + tf * tf * M3 * M3;
     Md[k] = DIR == 0 ? M1 : (DIR == 1 ? M2 : M3);
-    Pi[k] = (double)W[k][8];
+    Pi[k] = (F)W[k][8];
   }
   if (ePrev <= 0.1) {
 #pragma unroll
     for (int k = 0; k < N; ++k) e[k] = M0[k] - M[k] / M0[k];
   } else {
-    solve_e_lockstep<EOS, N>(T, ePrev, M0, M, Pi, e, diag);
+    solve_e_lockstep<EOS, N, F>(T, F(ePrev), M0, M, Pi, e, diag);
   }
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double ek = e[k];
+    F ek = e[k];
     if (isnan(ek)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
-    double p = eos_p<EOS>(T, ek);
-    if (ek < 1.e-7) {
-      ek = 1.e-7;
-      p = 1.e-7;
+    F p = eos_p_t<EOS, F>(T, ek);
+    if (ek < F(1.e-7)) {
+      ek = F(1.e-7);
+      p = F(1.e-7);
       atomicAdd(&diag[DIAG_E_CLAMPED], 1ull);
     }
-    const double P = p + Pi[k];
-    v[k] = ((ek + P > 0.0) ? Md[k] : -Md[k]) / fabs(M0[k] + P);
+    const F P = p + Pi[k];
+    v[k] = (double)(((ek + P > F(0)) ? Md[k] : -Md[k]) / fabs(M0[k] + P));
   }
 }
 
