@@ -296,16 +296,25 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   R Wst[4][9];
   R cdv[5];
   if constexpr (KEEP_STATES) {
-    // all four states of the nine inversion variables stay in registers
+    // all four states of the nine inversion variables stay in registers; the
+    // 45 stencil values are loaded in one batch (one memory round trip for
+    // the direction instead of one per variable)
+    R raw[9][5];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      if (a == 8 && !BULK) continue;
+      const int n = a < 8 ? a : 14;
+#pragma unroll
+      for (int m = 0; m < 5; ++m) raw[a][m] = ld(n, (m - 2) * st);
+    }
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       if (a == 8 && !BULK) {
         Wst[0][8] = Wst[1][8] = Wst[2][8] = Wst[3][8] = cdv[4] = R(0);
         continue;
       }
-      const int n = a < 8 ? a : 14;
-      const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
-              vp2 = ld(n, 2 * st);
+      const R vm2 = raw[a][0], vm1 = raw[a][1], v0 = raw[a][2], vp1 = raw[a][3],
+              vp2 = raw[a][4];
       const R D0 = slope_t<R>(vm1, v0, vp1);
       Wst[0][a] = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
       Wst[1][a] = v0 + D0 * R(0.5);
@@ -370,9 +379,23 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   const R aF = fmax(fabs(rLf), fabs(rRf));
   const R aB = fmax(fabs(rLb), fabs(rRb));
 
+  // the six spatial pi components, whose flux comes after that of the kept
+  // states: their 30 stencil values in one batch
+  constexpr bool FLUX_FROM_KEPT = KEEP_STATES && LOCK != 2;
+  R raw2[6][5];
+  if constexpr (FLUX_FROM_KEPT) {
+#pragma unroll
+    for (int n = 8; n < 14; ++n)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) raw2[n - 8][m] = ldp(n, (m - 2) * st);
+  }
+
   // directional sources: central differences of pi^{tau mu} and Pi and of
   // the three spatial pi components of this direction
-  auto cd = [&](int n) { return (ldp(n, st) - ldp(n, -st)) * hc; };
+  auto cd = [&](int n) {
+    if constexpr (FLUX_FROM_KEPT) return (raw2[n - 8][3] - raw2[n - 8][1]) * hc;
+    return (ldp(n, st) - ldp(n, -st)) * hc;
+  };
   const R cPi = cdv[4];
   R Sd0, Sd1, Sd2, Sd3;
   if constexpr (DIR == 0) {
@@ -401,7 +424,6 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   // the kept states give the flux of the nine inversion variables, unless
   // LOCK == 2, which rebuilds every state from the stencil after the
   // root-finds so that none stays live through them
-  constexpr bool FLUX_FROM_KEPT = KEEP_STATES && LOCK != 2;
   int n0 = 0;
   if constexpr (FLUX_FROM_KEPT) {
 #pragma unroll
@@ -417,19 +439,27 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     }
     n0 = 8;
   }
-  const int n1 = FLUX_FROM_KEPT ? 14 : NV;
-#pragma unroll 1
-  for (int n = n0; n < n1; ++n) {
-    const R vm2 = ld(n, -2 * st), vm1 = ld(n, -st), v0 = ld(n, 0), vp1 = ld(n, st),
-            vp2 = ld(n, 2 * st);
+  auto flux_of = [&](R vm2, R vm1, R v0, R vp1, R vp2) {
     const R D0 = slope_t<R>(vm1, v0, vp1);
     const R rf = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
     const R lf = v0 + D0 * R(0.5);
     const R rb = v0 - D0 * R(0.5);
     const R lb = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
-    R r = GVH_FLUXDIFF(rf, lf, rb, lb);
-    if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
-    acc[n * BLOCK] += (double)r * dt;
+    return GVH_FLUXDIFF(rf, lf, rb, lb);
+  };
+  if constexpr (FLUX_FROM_KEPT) {
+#pragma unroll
+    for (int n = 8; n < 14; ++n) {
+      const R *w = raw2[n - 8];
+      acc[n * BLOCK] += (double)flux_of(w[0], w[1], w[2], w[3], w[4]) * dt;
+    }
+  } else {
+#pragma unroll 1
+    for (int n = n0; n < NV; ++n) {
+      R r = flux_of(ld(n, -2 * st), ld(n, -st), ld(n, 0), ld(n, st), ld(n, 2 * st));
+      if (n < 4) r += n == 0 ? Sd0 : (n == 1 ? Sd1 : (n == 2 ? Sd2 : Sd3));
+      acc[n * BLOCK] += (double)r * dt;
+    }
   }
 #undef GVH_FLUXDIFF
 }
