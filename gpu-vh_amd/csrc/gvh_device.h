@@ -576,11 +576,16 @@ __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double
   } else {
     solve_e_lockstep<EOS, N, F>(T, F(ePrev), M0, M, Pi, e, diag);
   }
+  // the N pressure gathers first, so that they are in flight together (an
+  // atomic between them would order them)
+  F pk[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) pk[k] = eos_p_t<EOS, F>(T, e[k]);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     F ek = e[k];
     if (isnan(ek)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
-    F p = eos_p_t<EOS, F>(T, ek);
+    F p = pk[k];
     if (ek < F(1.e-7)) {
       ek = F(1.e-7);
       p = F(1.e-7);
