@@ -610,8 +610,12 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, R(un / ut), acc);
 
   if constexpr (STAGE2) {
-#pragma unroll 1
-    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + q_at(A.qn, n)) * 0.5;
+    // q^n of the convex average: all NV loads in flight together
+    double qn[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) qn[n] = q_at(A.qn, n);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (acc[n * BLOCK] + qn[n]) * 0.5;
   }
 
   // inferred variables with this cell's e as the guess
