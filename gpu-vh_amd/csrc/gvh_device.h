@@ -383,10 +383,9 @@ __device__ __forceinline__ double zeta_over_s(double T, double tpeak) {
 // e0 - f/fp with fp = 1 - num/den is evaluated as e0 - f den/(den - num),
 // and 1/e0 is shared: three divisions per iteration instead of five, equal
 // to the reference's expression up to rounding.
-template <int EOS, bool DAMPED, typename F = double>
-__device__ __forceinline__ bool newton_step(const EosTable &T, F &e0, F M0, F M, F Pi) {
-  F p, cs2;
-  eos_p_cs2_t<EOS, F>(T, e0, p, cs2);
+// One Newton update of e0 given p(e0) and c_s^2(e0).
+template <bool DAMPED, typename F>
+__device__ __forceinline__ bool newton_update(F &e0, F p, F cs2, F M0, F M, F Pi) {
   const F ie0 = rcp_t(e0);
   const F cst2 = p * ie0;
   const F A = fma(M0, F(1) - cst2, Pi);
@@ -402,6 +401,12 @@ __device__ __forceinline__ bool newton_step(const EosTable &T, F &e0, F M0, F M,
   const bool done = fabs(e - e0) <= F(0.001) * fabs(e);
   e0 = e;
   return done;
+}
+template <int EOS, bool DAMPED, typename F = double>
+__device__ __forceinline__ bool newton_step(const EosTable &T, F &e0, F M0, F M, F Pi) {
+  F p, cs2;
+  eos_p_cs2_t<EOS, F>(T, e0, p, cs2);
+  return newton_update<DAMPED, F>(e0, p, cs2, M0, M, Pi);
 }
 
 template <int EOS>
@@ -451,10 +456,14 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, con
     for (int k = 0; k < N; ++k) e[k] = ePrev;
     unsigned failed = 0;  // chains that reached a NaN iterate (see solve_e)
     for (int it = 0; it < MAX_ITERS / 2 && act; ++it) {
+      // the N table gathers first, all in flight together
+      F pk[N], ck[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) eos_p_cs2_t<EOS, F>(T, e[k], pk[k], ck[k]);
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         F ek = e[k];
-        const bool done = newton_step<EOS, false, F>(T, ek, M0[k], M[k], Pi[k]);
+        const bool done = newton_update<false, F>(ek, pk[k], ck[k], M0[k], M[k], Pi[k]);
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
@@ -465,10 +474,14 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, con
       }
     }
     for (int it = MAX_ITERS / 2; it < MAX_ITERS && act; ++it) {
+      // the N table gathers first, all in flight together
+      F pk[N], ck[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) eos_p_cs2_t<EOS, F>(T, e[k], pk[k], ck[k]);
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         F ek = e[k];
-        const bool done = newton_step<EOS, true, F>(T, ek, M0[k], M[k], Pi[k]);
+        const bool done = newton_update<true, F>(ek, pk[k], ck[k], M0[k], M[k], Pi[k]);
         const bool live = (act >> k) & 1u;
         e[k] = live ? ek : e[k];
         if (live && done) act &= ~(1u << k);
