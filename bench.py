@@ -93,6 +93,32 @@ def time_steps(ctx, args):
     return elapsed, s1, s2
 
 
+VALIDITY_BYTES_PER_CELL = 31 * 8  # reads e, p, u, u_prev, 10 pi, Pi; writes 10 values
+
+
+def time_validity(ctx, args, stream, ncells, reps=10):
+    """The validity kernel (gvh_check_validity) on the stepped state: HIP
+    events on the context's stream around reps launches."""
+    import torch
+
+    t = args.t0 + (args.warmup + args.steps + args.timed_steps) * args.dt
+    ctx.validity_enable()
+    ctx.check_validity(t)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        ctx.check_validity(t)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    gbs = VALIDITY_BYTES_PER_CELL * ncells / (ms * 1e-3) / 1e9
+    log("validity kernel: %.4f ms, %.0f GB/s algorithmic" % (ms, gbs))
+    return {"kernel": "validity_kernel (gvh_check_validity)", "ms": round(ms, 4),
+            "bytes_per_cell": VALIDITY_BYTES_PER_CELL, "bound": "hbm",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def measured_traffic(workload_key):
     """HBM bytes per step of the stage kernels from rocprofv3 FETCH_SIZE and
     WRITE_SIZE runs, calibrated against a copy kernel (profiles/traffic.json),
@@ -402,6 +428,7 @@ def main():
                                 [p for p in ("fp64", "mixed") if p != args.precision])
     res = {}
     diag = None
+    validity = None
     for form in forms:
         prec = gpuvh.FP64 if form == "fp64" else gpuvh.MIXED
         ctx = make_context(args, prec, eos, stream.cuda_stream)
@@ -409,6 +436,8 @@ def main():
         elapsed, s1, s2 = time_steps(ctx, args)
         if form == args.precision:
             diag = ctx.diagnostics()
+        if form == "fp64":
+            validity = time_validity(ctx, args, stream, ncells)
         ctx.close()
         bpc = algorithmic_bytes_per_cell_update(True, form)
         kt_ms = float(np.mean(s1) + np.mean(s2))
@@ -488,6 +517,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "validity": validity,
         "diagnostics": diag,
     }
     print(json.dumps(out))
