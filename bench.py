@@ -321,7 +321,7 @@ def run_serve_mode(args):
                         "steps_mean": round(float(np.mean([r["steps"] for r in done])), 1),
                         "frozen": int(sum(r["frozen"] for r in done)), "surface_rows": rows}
         log("serve %s: %s" % (name, report[name]))
-    out = {"metric": "events/s at 80%% of capacity, p50/p99 latency", "value": report["mixed"]["events_per_s"],
+    out = {"metric": "events/s at 80% of capacity, p50/p99 latency", "value": report["mixed"]["events_per_s"],
            "unit": "events/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32/f64 mixed",
            "data": "synthetic",
            "config": {"workload": "%d events, half 2+1D 121^2..201^2, half 3+1D 121^2..201^2 x 21..51, "
