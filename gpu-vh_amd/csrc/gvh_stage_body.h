@@ -290,6 +290,20 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   };
   auto ldp = [&](int n, long long o) -> R { return R((qP + (n - 4) * G + o)[s]); };
 
+  // the six spatial pi components, whose flux comes after that of the kept
+  // states: their 30 stencil values in one batch.  In FP32 they are loaded
+  // before the root-finds and stay in flight during them; in FP64 the 60
+  // registers this takes would cost a wave per SIMD, so they are loaded
+  // after the root-finds.
+  constexpr bool FLUX_FROM_KEPT = KEEP_STATES && LOCK != 2;
+  constexpr bool EARLY_RAW2 = FLUX_FROM_KEPT && sizeof(R) == 4;
+  R raw2[6][5];
+  auto load_raw2 = [&]() {
+#pragma unroll
+    for (int n = 8; n < 14; ++n)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) raw2[n - 8][m] = ldp(n, (m - 2) * st);
+  };
   double vRf, vLf, vRb, vLb;
   // the four states when KEEP_STATES: [0] right of i+1/2, [1] left of i+1/2,
   // [2] right of i-1/2, [3] left of i-1/2
@@ -322,6 +336,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
       Wst[3][a] = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
       if (a >= 4) cdv[a - 4] = (vp1 - vm1) * hc;
     }
+    if constexpr (EARLY_RAW2) load_raw2();
     double v4[4];
     if constexpr (LOCK) {
       flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, v4);
@@ -379,16 +394,8 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   const R aF = fmax(fabs(rLf), fabs(rRf));
   const R aB = fmax(fabs(rLb), fabs(rRb));
 
-  // the six spatial pi components, whose flux comes after that of the kept
-  // states: their 30 stencil values in one batch
-  constexpr bool FLUX_FROM_KEPT = KEEP_STATES && LOCK != 2;
-  R raw2[6][5];
-  if constexpr (FLUX_FROM_KEPT) {
-#pragma unroll
-    for (int n = 8; n < 14; ++n)
-#pragma unroll
-      for (int m = 0; m < 5; ++m) raw2[n - 8][m] = ldp(n, (m - 2) * st);
-  }
+
+  if constexpr (FLUX_FROM_KEPT && !EARLY_RAW2) load_raw2();
 
   // directional sources: central differences of pi^{tau mu} and Pi and of
   // the three spatial pi components of this direction
@@ -659,26 +666,24 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
     }
   }
 
-  // write the cell and, on the boundary, its two ghost layers
-  // (GhostCells.cu:15-144: zero-gradient copy, edges and corners untouched)
-  // offsets of this cell's ghost copies: two per lattice face it touches (a
-  // cell of a one-point axis touches both faces of that axis)
-  long long gh[12];
-  int ng = 0;
-  if (i == NG) { gh[ng++] = -1; gh[ng++] = -2; }
-  if (i == A.nx + 1) { gh[ng++] = 1; gh[ng++] = 2; }
-  if (j == NG) { gh[ng++] = -sy; gh[ng++] = -2 * sy; }
-  if (j == A.ny + 1) { gh[ng++] = sy; gh[ng++] = 2 * sy; }
-  if (k == NG) { gh[ng++] = -sz; gh[ng++] = -2 * sz; }
-  if (k == A.nz + 1) { gh[ng++] = sz; gh[ng++] = 2 * sz; }
-
   if constexpr (STAGE2) {
     if (A.reg) A.reg[s] = fs_rec;
   }
 
+  // write the cell and, on the boundary, its two ghost layers
+  // (GhostCells.cu:15-144: zero-gradient copy, edges and corners untouched);
+  // a cell of a one-point axis touches both faces of that axis
+  const bool gx0 = i == NG, gx1 = i == A.nx + 1, gy0 = j == NG, gy1 = j == A.ny + 1;
+  const bool gz0 = k == NG, gz1 = k == A.nz + 1;
+  const long long sy2 = 2 * sy, sz2 = 2 * sz;
   auto put = [&](auto *base, auto val) {
     base[s] = val;
-    for (int m = 0; m < ng; ++m) base[(long long)s + gh[m]] = val;
+    if (gx0) { base[(long long)s - 1] = val; base[(long long)s - 2] = val; }
+    if (gx1) { base[(long long)s + 1] = val; base[(long long)s + 2] = val; }
+    if (gy0) { base[(long long)s - sy] = val; base[(long long)s - sy2] = val; }
+    if (gy1) { base[(long long)s + sy] = val; base[(long long)s + sy2] = val; }
+    if (gz0) { base[(long long)s - sz] = val; base[(long long)s - sz2] = val; }
+    if (gz1) { base[(long long)s + sz] = val; base[(long long)s + sz2] = val; }
   };
   PS *qoutP = reinterpret_cast<PS *>(A.qout + 4 * G);
 #pragma unroll 1
