@@ -189,8 +189,11 @@ int rebuild(gvh_batch *b) {
   const bool mixed = b->opts.precision == GVH_MIXED;
   auto grid = [&](int n) { return std::max(8, std::min(n, b->grid_cap)); };
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total,
-                                     grid(b->ntiles_total), mixed, b->stream);
+  // the record is a copy with few registers: up to 8 waves per SIMD of it
+  // keep enough loads in flight to stream HBM
+  const int rec_grid = std::max(8, std::min(b->ntiles_total, 8 * b->grid_cap));
+  hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total, rec_grid, mixed,
+                                     b->stream);
   if (e == hipSuccess) e = launch_batch_check(b->d_slots, b->max_slots, b->stream);
   for (int stage = 1; stage <= 2 && e == hipSuccess; ++stage)
     for (const auto &g : b->groups) {
