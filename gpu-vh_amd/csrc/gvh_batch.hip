@@ -104,6 +104,11 @@ struct gvh_batch {
   gvh_options opts{};
   std::vector<double> tab_p, tab_cs2, tab_T;
   hipStream_t stream = nullptr;
+  // streams of the freeze-out window searches: the events whose window is
+  // due at a step are searched together, round-robin over these (default
+  // priority, like every stream here)
+  static constexpr int NSEARCH = 8;
+  hipStream_t search[NSEARCH] = {};
   size_t budget = 0, used = 0;
   int max_slots = 0, fofreq = 10, autotune = 0, grid_cap = 1024;
   BatchSlot *d_slots = nullptr;
@@ -422,6 +427,8 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
   b->autotune = B.autotune;
   b->origin = Clock::now();
   hipError_t e = hipStreamCreate(&b->stream);
+  for (hipStream_t &q : b->search)
+    if (e == hipSuccess) e = hipStreamCreate(&q);
   int ncu = 256;
   if (e == hipSuccess) {
     int dev = 0;
@@ -459,6 +466,8 @@ extern "C" int gvh_batch_destroy(gvh_batch *b) {
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->h_slots) (void)hipHostFree(b->h_slots);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (hipStream_t &q : b->search)
+    if (q) (void)hipStreamDestroy(q);
   delete b;
   return GVH_OK;
 }
@@ -508,21 +517,44 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
   HIP_TRY(hipMemcpyAsync(b->h_slots, b->d_slots, b->max_slots * sizeof(BatchSlot),
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  // the windows due at this step (loop index n = steps before it + 1):
+  // searched together, phase by phase, one wait per phase
+  std::vector<int> due;
+  for (int s = 0; s < b->max_slots; ++s) {
+    const Live &L = b->live[s];
+    const int n = L.steps + 1;
+    if (L.ctx && L.surface && n <= L.max_steps && n % b->fofreq == b->fofreq - 1) due.push_back(s);
+  }
+  if (!due.empty()) {
+    auto qs = [&](size_t k) { return b->search[k % gvh_batch::NSEARCH]; };
+    auto wait_all = [&]() -> int {
+      for (size_t k = 0; k < due.size() && k < (size_t)gvh_batch::NSEARCH; ++k)
+        HIP_TRY(hipStreamSynchronize(qs(k)));
+      return GVH_OK;
+    };
+    for (size_t k = 0; k < due.size(); ++k)
+      if ((rc = fo_find_count(b->live[due[k]].ctx, b->live[due[k]].steps + 1, qs(k)))) return rc;
+    if ((rc = wait_all())) return rc;
+    for (size_t k = 0; k < due.size(); ++k)
+      if ((rc = fo_find_elements(b->live[due[k]].ctx, qs(k)))) return rc;
+    if ((rc = wait_all())) return rc;
+    for (size_t k = 0; k < due.size(); ++k) {
+      Live &L = b->live[due[k]];
+      const BatchSlot &S = b->h_slots[due[k]];
+      long long ne = 0;
+      const double t = S.t0 + L.steps * S.A.dt;
+      if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne))) return rc;
+      const size_t old = L.rows.size();
+      L.rows.resize(old + (size_t)ne * GVH_FO_COLUMNS);
+      if ((rc = fo_rows_async(L.ctx, L.rows.data() + old, ne, qs(k)))) return rc;
+    }
+    if ((rc = wait_all())) return rc;
+  }
   running = 0;
   for (int s = 0; s < b->max_slots; ++s) {
     Live &L = b->live[s];
     if (!L.ctx) continue;
     const BatchSlot &S = b->h_slots[s];
-    // the window of loop index n = (steps before this batch step) + 1
-    const int n = L.steps + 1;
-    if (L.surface && n <= L.max_steps && n % b->fofreq == b->fofreq - 1) {
-      long long ne = 0;
-      const double t = S.t0 + L.steps * S.A.dt;
-      if ((rc = gvh_freezeout_find(L.ctx, n, t, &ne))) return rc;
-      const size_t old = L.rows.size();
-      L.rows.resize(old + (size_t)ne * GVH_FO_COLUMNS);
-      if ((rc = gvh_freezeout_rows(L.ctx, L.rows.data() + old, ne))) return rc;
-    }
     L.steps = S.steps;
     if (S.status == SLOT_DONE) {
       if ((rc = retire(b, s))) return rc;

@@ -81,6 +81,10 @@ struct FoState {
   size_t scan_cap = 0;
   unsigned *total = nullptr;  // device scalars: [0] scan total, [1] overflow
   long long ncubes = 0, nrows = 0;
+  // the window search in progress (fo_find_count .. fo_find_emit)
+  unsigned htot[2] = {0, 0};  // host copies of total[0], total[1]
+  long long nrow_pending = 0;
+  int n_pending = 0;
 };
 
 double *fo_ring(FoState *f) { return f ? f->ring : nullptr; }
@@ -631,9 +635,15 @@ extern "C" int gvh_freezeout_record(gvh_context *c, int n) {
   return GVH_OK;
 }
 
-extern "C" int gvh_freezeout_find(gvh_context *c, int n, double t, long long *nelements) {
-  if (!c || !c->fo) return gvh_fail(GVH_EINVAL, "freeze-out not initialised (gvh_freezeout_init)");
-  if (!nelements) return gvh_fail(GVH_EINVAL, "nelements is NULL");
+// The window search in three phases, each ending in a small device-to-host
+// copy that sizes the next: one context's search runs them back to back
+// (gvh_freezeout_find); a batch runs each phase for all events whose window
+// is due, on several streams, with one wait per phase.
+namespace gvh {
+
+// Phase 1: surface cubes per (it, ix, iy) row and their scan; the number of
+// cubes is copied to f->htot[0].
+int fo_find_count(gvh_context *c, int n, hipStream_t st) {
   FoState *f = c->fo;
   const int start = n <= f->fofreq ? 2 : 0;  // HydroPlugin.cpp:272-275
   Window W = make_window(c, f, start);
@@ -641,20 +651,34 @@ extern "C" int gvh_freezeout_find(gvh_context *c, int n, double t, long long *ne
   int rc;
   if ((rc = ensure_rows_per_window(f, (size_t)nrow)) || (rc = ensure_scan(f, (size_t)nrow)))
     return rc;
-  hipStream_t st = c->stream;
+  f->n_pending = n;
+  f->nrow_pending = nrow;
+  f->ncubes = 0;
+  f->nrows = 0;
   fo_rows_kernel<false><<<grid_for(nrow, 256), 256, 0, st>>>(W, f->rowcnt, nullptr, nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(scan_with_total(f->rowcnt, f->rowoff, (size_t)nrow, f->scan_tmp, f->total, st));
-  unsigned h[2] = {0, 0};
-  HIP_TRY(hipMemcpyAsync(h, f->total, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  const long long ncubes = h[0];
+  HIP_TRY(hipMemcpyAsync(f->htot, f->total, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  return GVH_OK;
+}
+
+// Phase 2 (phase 1 complete): the cube list in the reference's order, the
+// elements of each cube, and the scan of element counts; the number of rows
+// and the overflow count are copied to f->htot.
+int fo_find_elements(gvh_context *c, hipStream_t st) {
+  FoState *f = c->fo;
+  const long long ncubes = f->htot[0];
   f->ncubes = ncubes;
-  f->nrows = 0;
-  *nelements = 0;
-  if (ncubes == 0) return GVH_OK;
+  if (ncubes == 0) {
+    f->htot[0] = f->htot[1] = 0;
+    return GVH_OK;
+  }
+  const int start = f->n_pending <= f->fofreq ? 2 : 0;
+  Window W = make_window(c, f, start);
+  int rc;
   if ((rc = ensure_cubes(f, (size_t)ncubes)) || (rc = ensure_scan(f, (size_t)ncubes))) return rc;
-  fo_rows_kernel<true><<<grid_for(nrow, 256), 256, 0, st>>>(W, nullptr, f->rowoff, f->cubes);
+  fo_rows_kernel<true><<<grid_for(f->nrow_pending, 256), 256, 0, st>>>(W, nullptr, f->rowoff,
+                                                                        f->cubes);
   HIP_TRY(hipGetLastError());
   CellSource S{W, f->cubes, nullptr, true};
   HIP_TRY(hipMemsetAsync(f->total + 1, 0, sizeof(unsigned), st));
@@ -671,36 +695,72 @@ extern "C" int gvh_freezeout_find(gvh_context *c, int n, double t, long long *ne
   HIP_TRY(hipGetLastError());
   HIP_TRY(scan_with_total((const unsigned *)f->ecnt, f->eoff, (size_t)ncubes, f->scan_tmp,
                           f->total, st));
-  HIP_TRY(hipMemcpyAsync(h, f->total, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (h[1]) return gvh_fail(GVH_EINVAL, "%u freeze-out cells exceed %d elements", h[1], corn::MAXE);
-  const long long nrows = h[0];
+  HIP_TRY(hipMemcpyAsync(f->htot, f->total, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  return GVH_OK;
+}
+
+// Phase 3 (phase 2 complete): the rows of the elements into f->rows; their
+// number in *nelements.  t is the loop's time at this window.
+int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements) {
+  FoState *f = c->fo;
+  *nelements = 0;
+  if (f->ncubes == 0) return GVH_OK;
+  if (f->htot[1])
+    return gvh_fail(GVH_EINVAL, "%u freeze-out cells exceed %d elements", f->htot[1], corn::MAXE);
+  const long long nrows = f->htot[0];
+  int rc;
   if ((rc = ensure_out_rows(f, (size_t)nrows))) return rc;
+  const int start = f->n_pending <= f->fofreq ? 2 : 0;
   EmitArgs A{};
-  A.W = W;
+  A.W = make_window(c, f, start);
   A.cubes = f->cubes;
   A.ecnt = f->ecnt;
   A.eoff = f->eoff;
   A.elem = f->elem;
   A.rows = f->rows;
-  A.ncubes = ncubes;
+  A.ncubes = f->ncubes;
   A.t = t;
   A.t0 = c->Hp.initialProperTimePoint;
-  A.dt = dt;
-  A.dx = dx;
-  A.dy = dy;
-  A.dz = dz;
-  A.n = n;
+  A.dt = c->L.latticeSpacingProperTime;
+  A.dx = c->L.latticeSpacingX;
+  A.dy = c->L.latticeSpacingY;
+  A.dz = c->L.latticeSpacingRapidity;
+  A.n = f->n_pending;
   A.fofreq = f->fofreq;
   A.flags = f->flags;
   A.eos = c->eos.kind;
   A.tab = c->tab;
-  fo_emit_kernel<<<grid_for(ncubes, 128), 128, 0, st>>>(A);
+  fo_emit_kernel<<<grid_for(f->ncubes, 128), 128, 0, st>>>(A);
   HIP_TRY(hipGetLastError());
   f->nrows = nrows;
   *nelements = nrows;
   return GVH_OK;
 }
+
+}  // namespace gvh
+
+extern "C" int gvh_freezeout_find(gvh_context *c, int n, double t, long long *nelements) {
+  if (!c || !c->fo) return gvh_fail(GVH_EINVAL, "freeze-out not initialised (gvh_freezeout_init)");
+  if (!nelements) return gvh_fail(GVH_EINVAL, "nelements is NULL");
+  *nelements = 0;
+  hipStream_t st = c->stream;
+  int rc;
+  if ((rc = fo_find_count(c, n, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  if ((rc = fo_find_elements(c, st))) return rc;
+  HIP_TRY(hipStreamSynchronize(st));
+  return fo_find_emit(c, t, st, nelements);
+}
+
+// Rows of the last search on stream st (asynchronous).
+namespace gvh {
+int fo_rows_async(gvh_context *c, double *rows, long long nrows, hipStream_t st) {
+  if (nrows == 0) return GVH_OK;
+  HIP_TRY(hipMemcpyAsync(rows, c->fo->rows, (size_t)nrows * NCOL * sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+  return GVH_OK;
+}
+}  // namespace gvh
 
 extern "C" int gvh_freezeout_rows(gvh_context *c, double *rows, long long nrows) {
   if (!c || !c->fo) return gvh_fail(GVH_EINVAL, "freeze-out not initialised (gvh_freezeout_init)");

@@ -41,6 +41,12 @@ StageArgs make_stage_args(gvh_context *c, int stage, double t);
 Buffers context_buffers(const gvh_context *c);
 Roles context_roles(const gvh_context *c);
 double *fo_ring(FoState *f);
+// The window search in phases (gvh_freezeout.hip): each phase may start once
+// the previous one's stream work is complete.
+int fo_find_count(gvh_context *c, int n, hipStream_t st);
+int fo_find_elements(gvh_context *c, hipStream_t st);
+int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements);
+int fo_rows_async(gvh_context *c, double *rows, long long nrows, hipStream_t st);
 }  // namespace gvh
 
 #define HIP_TRY(expr)                                                              \
