@@ -25,6 +25,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <vector>
 
@@ -58,13 +59,35 @@ struct Pending {
   size_t footprint;
 };
 
+// Surface rows of an event, one uninitialised block per window (no zero
+// fill, no reallocation as an event's surface grows).
+struct Rows {
+  std::vector<std::unique_ptr<double[]>> blocks;
+  std::vector<long long> counts;  // rows per block
+  long long total = 0;
+  double *append(long long n) {
+    blocks.emplace_back(new double[(size_t)n * GVH_FO_COLUMNS]);
+    counts.push_back(n);
+    total += n;
+    return blocks.back().get();
+  }
+  void copy_to(double *out, long long n) const {
+    for (size_t b = 0; b < blocks.size() && n > 0; ++b) {
+      const long long m = std::min(n, counts[b]);
+      std::memcpy(out, blocks[b].get(), (size_t)m * GVH_FO_COLUMNS * sizeof(double));
+      out += (size_t)m * GVH_FO_COLUMNS;
+      n -= m;
+    }
+  }
+};
+
 struct Live {
   long long id = -1;
   gvh_context *ctx = nullptr;
   int surface = 0, keep = 0;
   double t_submit = 0, t_admit = 0;
   size_t footprint = 0;
-  std::vector<double> rows;
+  Rows rows;
   int steps = 0;       // steps at the last poll
   int max_steps = 0;
   int has_z = 0, lock = 0, ntiles = 0;
@@ -73,7 +96,7 @@ struct Live {
 struct Finished {
   gvh_event_result r;
   gvh_context *ctx = nullptr;
-  std::vector<double> rows;
+  Rows rows;
   size_t footprint = 0;
 };
 
@@ -367,11 +390,11 @@ int retire(gvh_batch *b, int s) {
   F.r.steps = S.steps;
   F.r.frozen = S.frozen_stop;
   F.r.t_final = S.t0 + S.steps * S.A.dt;
-  F.r.surface_rows = (long long)(L.rows.size() / GVH_FO_COLUMNS);
+  F.r.surface_rows = L.rows.total;
   F.r.t_submit = L.t_submit;
   F.r.t_admit = L.t_admit;
   F.r.t_done = seconds(b->origin, Clock::now());
-  F.rows.swap(L.rows);
+  F.rows = std::move(L.rows);
   if (L.keep) {
     F.ctx = c;
     F.footprint = L.footprint;
@@ -544,9 +567,7 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
       long long ne = 0;
       const double t = S.t0 + L.steps * S.A.dt;
       if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne))) return rc;
-      const size_t old = L.rows.size();
-      L.rows.resize(old + (size_t)ne * GVH_FO_COLUMNS);
-      if ((rc = fo_rows_async(L.ctx, L.rows.data() + old, ne, qs(k)))) return rc;
+      if (ne > 0 && (rc = fo_rows_async(L.ctx, L.rows.append(ne), ne, qs(k)))) return rc;
     }
     if ((rc = wait_all())) return rc;
   }
@@ -592,11 +613,11 @@ extern "C" int gvh_batch_rows(gvh_batch *b, long long id, double *rows, long lon
   if (!b) return gvh_fail(GVH_EINVAL, "batch is NULL");
   auto it = b->done.find(id);
   if (it == b->done.end()) return gvh_fail(GVH_EINVAL, "event %lld has not finished", id);
-  const long long have = (long long)(it->second.rows.size() / GVH_FO_COLUMNS);
+  const long long have = it->second.rows.total;
   if (nrows < 0 || nrows > have)
     return gvh_fail(GVH_EINVAL, "event %lld has %lld rows, asked for %lld", id, have, nrows);
   if (nrows && !rows) return gvh_fail(GVH_EINVAL, "rows is NULL");
-  std::copy(it->second.rows.begin(), it->second.rows.begin() + nrows * GVH_FO_COLUMNS, rows);
+  it->second.rows.copy_to(rows, nrows);
   return GVH_OK;
 }
 
