@@ -21,11 +21,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -132,6 +134,10 @@ struct gvh_batch {
   // priority, like every stream here)
   static constexpr int NSEARCH = 8;
   hipStream_t search[NSEARCH] = {};
+  // pinned staging of the rows of each due window (device -> host at full
+  // PCIe rate), then copied into the event's row blocks by host threads
+  std::vector<double *> stage;
+  std::vector<size_t> stage_cap;  // doubles
   size_t budget = 0, used = 0;
   int max_slots = 0, fofreq = 10, autotune = 0, grid_cap = 1024;
   BatchSlot *d_slots = nullptr;
@@ -156,6 +162,29 @@ struct gvh_batch {
 };
 
 namespace {
+
+// Host copies dst[i][0..len[i]) = src[i][...], cut into pieces of at most
+// 4 MB spread over up to 8 threads.
+void parallel_copy(const std::vector<std::pair<double *, const double *>> &jobs,
+                   const std::vector<size_t> &len) {
+  constexpr size_t PIECE = (4u << 20) / sizeof(double);
+  std::vector<std::tuple<double *, const double *, size_t>> pieces;
+  for (size_t i = 0; i < jobs.size(); ++i)
+    for (size_t o = 0; o < len[i]; o += PIECE)
+      pieces.emplace_back(jobs[i].first + o, jobs[i].second + o, std::min(PIECE, len[i] - o));
+  if (pieces.empty()) return;
+  const size_t nt = std::min<size_t>({8, pieces.size(), std::max(1u, std::thread::hardware_concurrency())});
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t i = next++; i < pieces.size(); i = next++)
+      std::memcpy(std::get<0>(pieces[i]), std::get<1>(pieces[i]),
+                  std::get<2>(pieces[i]) * sizeof(double));
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
 
 int grow_tiles(gvh_batch *b, size_t n) {
   if (n <= b->tile_cap) return GVH_OK;
@@ -491,6 +520,8 @@ extern "C" int gvh_batch_destroy(gvh_batch *b) {
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (hipStream_t &q : b->search)
     if (q) (void)hipStreamDestroy(q);
+  for (double *p : b->stage)
+    if (p) (void)hipHostFree(p);
   delete b;
   return GVH_OK;
 }
@@ -561,15 +592,37 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
     for (size_t k = 0; k < due.size(); ++k)
       if ((rc = fo_find_elements(b->live[due[k]].ctx, qs(k)))) return rc;
     if ((rc = wait_all())) return rc;
+    if (b->stage.size() < due.size()) {
+      b->stage.resize(due.size(), nullptr);
+      b->stage_cap.resize(due.size(), 0);
+    }
+    std::vector<long long> ne(due.size(), 0);
     for (size_t k = 0; k < due.size(); ++k) {
       Live &L = b->live[due[k]];
       const BatchSlot &S = b->h_slots[due[k]];
-      long long ne = 0;
       const double t = S.t0 + L.steps * S.A.dt;
-      if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne))) return rc;
-      if (ne > 0 && (rc = fo_rows_async(L.ctx, L.rows.append(ne), ne, qs(k)))) return rc;
+      if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne[k]))) return rc;
+      const size_t need = (size_t)ne[k] * GVH_FO_COLUMNS;
+      if (need > b->stage_cap[k]) {
+        if (b->stage[k]) HIP_TRY(hipHostFree(b->stage[k]));
+        b->stage[k] = nullptr;
+        b->stage_cap[k] = 0;
+        const size_t cap = need + need / 2;
+        HIP_TRY(hipHostMalloc((void **)&b->stage[k], cap * sizeof(double)));
+        b->stage_cap[k] = cap;
+      }
+      if (ne[k] > 0 && (rc = fo_rows_async(L.ctx, b->stage[k], ne[k], qs(k)))) return rc;
     }
     if ((rc = wait_all())) return rc;
+    // staging -> the events' row blocks, split over host threads
+    std::vector<std::pair<double *, const double *>> jobs;
+    std::vector<size_t> len;
+    for (size_t k = 0; k < due.size(); ++k)
+      if (ne[k] > 0) {
+        jobs.emplace_back(b->live[due[k]].rows.append(ne[k]), b->stage[k]);
+        len.push_back((size_t)ne[k] * GVH_FO_COLUMNS);
+      }
+    parallel_copy(jobs, len);
   }
   running = 0;
   for (int s = 0; s < b->max_slots; ++s) {
