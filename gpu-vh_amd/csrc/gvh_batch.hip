@@ -509,6 +509,8 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
 extern "C" int gvh_batch_destroy(gvh_batch *b) {
   if (!b) return GVH_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (hipStream_t q : b->search)  // a search left by an error path
+    if (q) (void)hipStreamSynchronize(q);
   destroy_graph(b);
   for (Live &L : b->live)
     if (L.ctx) gvh_destroy(L.ctx);
