@@ -83,8 +83,9 @@ def test_mixed_against_fp64(name, case, nsteps, variant, monkeypatch):
 def test_mixed_freezeout_normals():
     """Freeze-out over three windows of a 3+1D table-EoS event, iso-energy
     density halfway between the event's minimum and maximum: the mixed form
-    finds the FP64 form's cells in the same order, and the normals agree
-    within 1e-4 of the largest normal."""
+    finds the FP64 form's cells in the same order, the normals agree within
+    1e-4 of the largest normal, and e and u^mu interpolated on the surface
+    (the rows' columns 8-11) within 1e-4 of their largest value."""
     case = Case(25, 25, 11, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=gpuvh.EOS_TABLE,
                 zeta_tpeak=0.16 / gpuvh.HBARC)
     state = hotspot_state(case, seed=9, n_hot=12, amp=8.0)
@@ -115,3 +116,9 @@ def test_mixed_freezeout_normals():
     say("  max position difference %.2e, normal error / max normal %.2e" % (pos, err))
     assert pos <= 1e-4 * 0.1
     assert err <= MIXED_TOL
+    # u^x, u^y, u^eta, e on the surface
+    for col, name in ((8, "ux"), (9, "uy"), (10, "un"), (11, "e")):
+        scale = np.abs(a[:, col]).max()
+        d = np.abs(a[:, col] - b[:, col]).max() / scale
+        say("  surface %s: max |mixed - fp64| / max %.2e" % (name, d))
+        assert d <= MIXED_TOL, name
