@@ -154,8 +154,12 @@ struct gvh_batch {
   std::vector<Group> groups;
   int ntiles_total = 0;
   bool dirty = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // a step is two graphs: the record of the history slice (and the
+  // termination check), then the stages; window searches wait only for the
+  // first, so they run beside the stages
+  hipGraph_t graph = nullptr, graph2 = nullptr;
+  hipGraphExec_t exec = nullptr, exec2 = nullptr;
+  hipEvent_t recorded = nullptr;
   long long next_id = 0;
   Clock::time_point origin;
   std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;  // shape class -> (variant, strip)
@@ -208,8 +212,10 @@ int tiles_of(const gvh_context *c) {
 void destroy_graph(gvh_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+  if (b->exec2) (void)hipGraphExecDestroy(b->exec2);
+  if (b->graph2) (void)hipGraphDestroy(b->graph2);
+  b->exec = b->exec2 = nullptr;
+  b->graph = b->graph2 = nullptr;
 }
 
 // Tile map and per-slot tile ranges for the current running set, then the
@@ -242,34 +248,45 @@ int rebuild(gvh_batch *b) {
   HIP_TRY(hipMemcpyAsync(b->d_tiles, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice,
                          b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  // capture: record, check, stage 1 and 2 per group, advance
+  // capture two graphs: record and check; stage 1 and 2 per group and advance
   const bool mixed = b->opts.precision == GVH_MIXED;
   auto grid = [&](int n) { return std::max(8, std::min(n, b->grid_cap)); };
-  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  // the record is a copy with few registers: up to 8 waves per SIMD of it
-  // keep enough loads in flight to stream HBM
-  const int rec_grid = std::max(8, std::min(b->ntiles_total, 8 * b->grid_cap));
-  hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total, rec_grid, mixed,
-                                     b->stream);
-  if (e == hipSuccess) e = launch_batch_check(b->d_slots, b->max_slots, b->stream);
-  for (int stage = 1; stage <= 2 && e == hipSuccess; ++stage)
-    for (const auto &g : b->groups) {
-      e = launch_batch_stage(b->d_slots, b->d_tiles, g.first, g.first + g.ntiles, grid(g.ntiles),
-                             b->opts.eos, b->opts.bulk != 0, stage == 2, g.has_z != 0,
-                             g.lock != 0, mixed, b->stream);
-      if (e != hipSuccess) break;
+  auto capture = [&](auto &&body, hipGraph_t &graph, hipGraphExec_t &exec) -> int {
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = body();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return gvh_fail(GVH_EHIP, "batch step capture: %s",
+                      hipGetErrorString(e != hipSuccess ? e : e2));
     }
-  if (e == hipSuccess) e = launch_batch_advance(b->d_slots, b->max_slots, b->stream);
-  hipGraph_t g = nullptr;
-  hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-  if (e != hipSuccess || e2 != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    return gvh_fail(GVH_EHIP, "batch step capture: %s",
-                    hipGetErrorString(e != hipSuccess ? e : e2));
-  }
-  b->graph = g;
-  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  return GVH_OK;
+    graph = g;
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return GVH_OK;
+  };
+  rc = capture([&]() {
+    // the record is a copy with few registers: up to 8 waves per SIMD of it
+    // keep enough loads in flight to stream HBM
+    const int rec_grid = std::max(8, std::min(b->ntiles_total, 8 * b->grid_cap));
+    hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total, rec_grid, mixed,
+                                       b->stream);
+    if (e == hipSuccess) e = launch_batch_check(b->d_slots, b->max_slots, b->stream);
+    return e;
+  }, b->graph, b->exec);
+  if (rc) return rc;
+  return capture([&]() {
+    hipError_t e = hipSuccess;
+    for (int stage = 1; stage <= 2 && e == hipSuccess; ++stage)
+      for (const auto &g : b->groups) {
+        e = launch_batch_stage(b->d_slots, b->d_tiles, g.first, g.first + g.ntiles,
+                               grid(g.ntiles), b->opts.eos, b->opts.bulk != 0, stage == 2,
+                               g.has_z != 0, g.lock != 0, mixed, b->stream);
+        if (e != hipSuccess) break;
+      }
+    if (e == hipSuccess) e = launch_batch_advance(b->d_slots, b->max_slots, b->stream);
+    return e;
+  }, b->graph2, b->exec2);
 }
 
 // Time (variant, strip) candidates on the context's current state, restoring
@@ -481,6 +498,7 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
   hipError_t e = hipStreamCreate(&b->stream);
   for (hipStream_t &q : b->search)
     if (e == hipSuccess) e = hipStreamCreate(&q);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->recorded, hipEventDisableTiming);
   int ncu = 256;
   if (e == hipSuccess) {
     int dev = 0;
@@ -524,6 +542,7 @@ extern "C" int gvh_batch_destroy(gvh_batch *b) {
     if (q) (void)hipStreamDestroy(q);
   for (double *p : b->stage)
     if (p) (void)hipHostFree(p);
+  if (b->recorded) (void)hipEventDestroy(b->recorded);
   delete b;
   return GVH_OK;
 }
@@ -569,18 +588,20 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
     if (nrunning) *nrunning = 0;
     return GVH_OK;
   }
-  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-  HIP_TRY(hipMemcpyAsync(b->h_slots, b->d_slots, b->max_slots * sizeof(BatchSlot),
-                         hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  // the windows due at this step (loop index n = steps before it + 1):
-  // searched together, phase by phase, one wait per phase
+  // the windows due at this step (loop index n = steps before it + 1): their
+  // history is complete once the record graph has run; they are searched
+  // together, phase by phase, on the search streams while the stages run
   std::vector<int> due;
   for (int s = 0; s < b->max_slots; ++s) {
     const Live &L = b->live[s];
     const int n = L.steps + 1;
     if (L.ctx && L.surface && n <= L.max_steps && n % b->fofreq == b->fofreq - 1) due.push_back(s);
   }
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (!due.empty()) HIP_TRY(hipEventRecord(b->recorded, b->stream));
+  HIP_TRY(hipGraphLaunch(b->exec2, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->h_slots, b->d_slots, b->max_slots * sizeof(BatchSlot),
+                         hipMemcpyDeviceToHost, b->stream));
   if (!due.empty()) {
     auto qs = [&](size_t k) { return b->search[k % gvh_batch::NSEARCH]; };
     auto wait_all = [&]() -> int {
@@ -588,6 +609,8 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
         HIP_TRY(hipStreamSynchronize(qs(k)));
       return GVH_OK;
     };
+    for (size_t k = 0; k < due.size() && k < (size_t)gvh_batch::NSEARCH; ++k)
+      HIP_TRY(hipStreamWaitEvent(qs(k), b->recorded, 0));
     for (size_t k = 0; k < due.size(); ++k)
       if ((rc = fo_find_count(b->live[due[k]].ctx, b->live[due[k]].steps + 1, qs(k)))) return rc;
     if ((rc = wait_all())) return rc;
@@ -601,8 +624,8 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
     std::vector<long long> ne(due.size(), 0);
     for (size_t k = 0; k < due.size(); ++k) {
       Live &L = b->live[due[k]];
-      const BatchSlot &S = b->h_slots[due[k]];
-      const double t = S.t0 + L.steps * S.A.dt;
+      // (the slot mirror is being copied back meanwhile: t from the context)
+      const double t = L.ctx->Hp.initialProperTimePoint + L.steps * L.ctx->L.latticeSpacingProperTime;
       if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne[k]))) return rc;
       const size_t need = (size_t)ne[k] * GVH_FO_COLUMNS;
       if (need > b->stage_cap[k]) {
@@ -626,6 +649,7 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
       }
     parallel_copy(jobs, len);
   }
+  HIP_TRY(hipStreamSynchronize(b->stream));
   running = 0;
   for (int s = 0; s < b->max_slots; ++s) {
     Live &L = b->live[s];
