@@ -49,7 +49,7 @@ using namespace gvh;
 namespace {
 
 using Clock = std::chrono::steady_clock;
-constexpr int TILE = 256;  // cells per tile (the stage kernel's block)
+constexpr int TILE = GVH_BLOCK;  // cells per tile (the stage kernel's block)
 
 struct Pending {
   long long id;
@@ -504,7 +504,8 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    b->grid_cap = std::max(8, ncu * 4 / 8 * 8);
+    // persistent blocks: 16 waves per CU of the stage kernel's blocks
+    b->grid_cap = std::max(8, ncu * 4 * (256 / GVH_BLOCK) / 8 * 8);
   }
   if (e == hipSuccess) e = hipMalloc((void **)&b->d_slots, b->max_slots * sizeof(BatchSlot));
   if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_slots, b->max_slots * sizeof(BatchSlot));

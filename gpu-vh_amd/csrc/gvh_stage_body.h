@@ -259,7 +259,7 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
 // Threads per block of the stage kernel; the per-thread accumulator of the 15
 // conserved variables lives in LDS as acc[n * BLOCK + threadIdx.x] so that the
 // variable loops can stay rolled (dynamic index) and registers stay few.
-constexpr int BLOCK = 256;
+constexpr int BLOCK = GVH_BLOCK;
 
 // One direction of eulerStepKernel{X,Y,Z}_1D: the two KT interface fluxes of
 // flux() (SemiDiscreteKurganovTadmorScheme.cu:16-57), each with both
