@@ -36,7 +36,7 @@
 
 namespace gvh {
 hipError_t launch_batch_stage(const BatchSlot *slots, const int *tile_slot, int begin, int end,
-                              int grid, int eos, bool bulk, bool stage2, bool has_z, bool lock,
+                              int eos, bool bulk, bool stage2, bool has_z, bool lock,
                               bool mixed, hipStream_t st);
 hipError_t launch_batch_record(BatchSlot *slots, const int *tile_slot, int ntiles, int grid,
                                bool mixed, hipStream_t st);
@@ -250,7 +250,6 @@ int rebuild(gvh_batch *b) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   // capture two graphs: record and check; stage 1 and 2 per group and advance
   const bool mixed = b->opts.precision == GVH_MIXED;
-  auto grid = [&](int n) { return std::max(8, std::min(n, b->grid_cap)); };
   auto capture = [&](auto &&body, hipGraph_t &graph, hipGraphExec_t &exec) -> int {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = body();
@@ -280,7 +279,7 @@ int rebuild(gvh_batch *b) {
     for (int stage = 1; stage <= 2 && e == hipSuccess; ++stage)
       for (const auto &g : b->groups) {
         e = launch_batch_stage(b->d_slots, b->d_tiles, g.first, g.first + g.ntiles,
-                               grid(g.ntiles), b->opts.eos, b->opts.bulk != 0, stage == 2,
+                               b->opts.eos, b->opts.bulk != 0, stage == 2,
                                g.has_z != 0, g.lock != 0, mixed, b->stream);
         if (e != hipSuccess) break;
       }

@@ -12,25 +12,24 @@ namespace gvh {
 // tile is one block's worth of cells of one event, cut exactly as that event
 // alone would be cut, so every event gets the arithmetic of a launch of its
 // own.  The stage pointers and times come from the slot's roles and step
-// count.  No waves-per-SIMD bound: at three waves the FP32 form ran 6%
-// slower over a 64-event batch than free, and the FP64 form's time did not
-// change at two.
+// count.  One block per tile (no tile loop): the slot is read before any
+// store of the block, so its fields stay in scalar registers; a loop over
+// tiles made the compiler reload them into vector registers, which cost a
+// wave per SIMD.
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool KEEP, int LOCK, typename R,
           typename PS>
-__global__ __launch_bounds__(BLOCK) void kt_batch_kernel(const BatchSlot *__restrict__ slots,
-                                                         const int *__restrict__ tile_slot,
-                                                         int begin, int end) {
+__global__ __launch_bounds__(BLOCK, stage_waves_per_simd<R>()) void kt_batch_kernel(
+    const BatchSlot *__restrict__ slots, const int *__restrict__ tile_slot, int begin) {
   __shared__ double acc_lds[15 * BLOCK];
-  for (int t = begin + (int)blockIdx.x; t < end; t += gridDim.x) {
-    const BatchSlot &S = slots[tile_slot[t]];
-    if (S.status != SLOT_RUN) continue;
-    StageArgs A = S.A;
-    stage_pointers(A, S.B, S.R, STAGE2 ? 2 : 1);
-    stage_times(A, S.t0 + S.steps * S.A.dt, STAGE2 ? 2 : 1);
-    const CellPos c = cell_of_tile(A, (unsigned)(t - S.tile_first), (unsigned)S.ntiles);
-    if (c.valid)
-      stage_cell<EOS, BULK, STAGE2, HAS_Z, true, KEEP, LOCK, R, PS>(A, c, acc_lds + threadIdx.x);
-  }
+  const int t = begin + (int)blockIdx.x;
+  const BatchSlot &S = slots[tile_slot[t]];
+  if (S.status != SLOT_RUN) return;
+  StageArgs A = S.A;
+  stage_pointers(A, S.B, S.R, STAGE2 ? 2 : 1);
+  stage_times(A, S.t0 + S.steps * S.A.dt, STAGE2 ? 2 : 1);
+  const CellPos c = cell_of_tile(A, (unsigned)(t - S.tile_first), (unsigned)S.ntiles);
+  if (c.valid)
+    stage_cell<EOS, BULK, STAGE2, HAS_Z, true, KEEP, LOCK, R, PS>(A, c, acc_lds + threadIdx.x);
 }
 
 // The run loop's per-step history update (FreezeOut.cpp:49-126) for every
@@ -118,44 +117,44 @@ __global__ void batch_advance_kernel(BatchSlot *__restrict__ slots, int nslots) 
 }
 
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool LOCK, bool MIXED>
-static void launch_b(const BatchSlot *slots, const int *tile_slot, int begin, int end, int grid,
+static void launch_b(const BatchSlot *slots, const int *tile_slot, int begin, int end,
                      hipStream_t st) {
   if constexpr (MIXED)
     hipLaunchKernelGGL((kt_batch_kernel<EOS, BULK, STAGE2, HAS_Z, true, LOCK ? 1 : 0, float, float>),
-                       dim3(grid), dim3(BLOCK), 0, st, slots, tile_slot, begin, end);
+                       dim3(end - begin), dim3(BLOCK), 0, st, slots, tile_slot, begin);
   else
     hipLaunchKernelGGL((kt_batch_kernel<EOS, BULK, STAGE2, HAS_Z, true, LOCK ? 1 : 0, double, double>),
-                       dim3(grid), dim3(BLOCK), 0, st, slots, tile_slot, begin, end);
+                       dim3(end - begin), dim3(BLOCK), 0, st, slots, tile_slot, begin);
 }
 
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool LOCK>
-static void launch_bm(const BatchSlot *slots, const int *tile_slot, int begin, int end, int grid,
+static void launch_bm(const BatchSlot *slots, const int *tile_slot, int begin, int end,
                       bool mixed, hipStream_t st) {
-  if (mixed) launch_b<EOS, BULK, STAGE2, HAS_Z, LOCK, true>(slots, tile_slot, begin, end, grid, st);
-  else launch_b<EOS, BULK, STAGE2, HAS_Z, LOCK, false>(slots, tile_slot, begin, end, grid, st);
+  if (mixed) launch_b<EOS, BULK, STAGE2, HAS_Z, LOCK, true>(slots, tile_slot, begin, end, st);
+  else launch_b<EOS, BULK, STAGE2, HAS_Z, LOCK, false>(slots, tile_slot, begin, end, st);
 }
 
 template <int EOS, bool BULK, bool STAGE2>
-static void launch_bz(const BatchSlot *slots, const int *tile_slot, int begin, int end, int grid,
+static void launch_bz(const BatchSlot *slots, const int *tile_slot, int begin, int end,
                       bool has_z, bool lock, bool mixed, hipStream_t st) {
   if (has_z) {
-    if (lock) launch_bm<EOS, BULK, STAGE2, true, true>(slots, tile_slot, begin, end, grid, mixed, st);
-    else launch_bm<EOS, BULK, STAGE2, true, false>(slots, tile_slot, begin, end, grid, mixed, st);
+    if (lock) launch_bm<EOS, BULK, STAGE2, true, true>(slots, tile_slot, begin, end, mixed, st);
+    else launch_bm<EOS, BULK, STAGE2, true, false>(slots, tile_slot, begin, end, mixed, st);
   } else {
-    if (lock) launch_bm<EOS, BULK, STAGE2, false, true>(slots, tile_slot, begin, end, grid, mixed, st);
-    else launch_bm<EOS, BULK, STAGE2, false, false>(slots, tile_slot, begin, end, grid, mixed, st);
+    if (lock) launch_bm<EOS, BULK, STAGE2, false, true>(slots, tile_slot, begin, end, mixed, st);
+    else launch_bm<EOS, BULK, STAGE2, false, false>(slots, tile_slot, begin, end, mixed, st);
   }
 }
 
 template <int EOS>
-static void launch_be(const BatchSlot *slots, const int *tile_slot, int begin, int end, int grid,
+static void launch_be(const BatchSlot *slots, const int *tile_slot, int begin, int end,
                       bool bulk, bool stage2, bool has_z, bool lock, bool mixed, hipStream_t st) {
   if (bulk) {
-    if (stage2) launch_bz<EOS, true, true>(slots, tile_slot, begin, end, grid, has_z, lock, mixed, st);
-    else launch_bz<EOS, true, false>(slots, tile_slot, begin, end, grid, has_z, lock, mixed, st);
+    if (stage2) launch_bz<EOS, true, true>(slots, tile_slot, begin, end, has_z, lock, mixed, st);
+    else launch_bz<EOS, true, false>(slots, tile_slot, begin, end, has_z, lock, mixed, st);
   } else {
-    if (stage2) launch_bz<EOS, false, true>(slots, tile_slot, begin, end, grid, has_z, lock, mixed, st);
-    else launch_bz<EOS, false, false>(slots, tile_slot, begin, end, grid, has_z, lock, mixed, st);
+    if (stage2) launch_bz<EOS, false, true>(slots, tile_slot, begin, end, has_z, lock, mixed, st);
+    else launch_bz<EOS, false, false>(slots, tile_slot, begin, end, has_z, lock, mixed, st);
   }
 }
 
@@ -163,18 +162,18 @@ static void launch_be(const BatchSlot *slots, const int *tile_slot, int begin, i
 // map; those events share the equation of state, bulk setting,
 // dimensionality, arrangement and precision.
 hipError_t launch_batch_stage(const BatchSlot *slots, const int *tile_slot, int begin, int end,
-                              int grid, int eos, bool bulk, bool stage2, bool has_z, bool lock,
+                              int eos, bool bulk, bool stage2, bool has_z, bool lock,
                               bool mixed, hipStream_t st) {
   switch (eos) {
     case EOS_CONFORMAL:
-      launch_be<EOS_CONFORMAL>(slots, tile_slot, begin, end, grid, bulk, stage2, has_z, lock, mixed,
+      launch_be<EOS_CONFORMAL>(slots, tile_slot, begin, end, bulk, stage2, has_z, lock, mixed,
                                st);
       break;
     case EOS_WB:
-      launch_be<EOS_WB>(slots, tile_slot, begin, end, grid, bulk, stage2, has_z, lock, mixed, st);
+      launch_be<EOS_WB>(slots, tile_slot, begin, end, bulk, stage2, has_z, lock, mixed, st);
       break;
     default:
-      launch_be<EOS_TABLE>(slots, tile_slot, begin, end, grid, bulk, stage2, has_z, lock, mixed, st);
+      launch_be<EOS_TABLE>(slots, tile_slot, begin, end, bulk, stage2, has_z, lock, mixed, st);
   }
   return hipGetLastError();
 }
