@@ -503,7 +503,7 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // persistent blocks: 16 waves per CU of the stage kernel's blocks
+    // 16 waves per CU of BLOCK-thread blocks (the record kernel takes 8x)
     b->grid_cap = std::max(8, ncu * 4 * (256 / GVH_BLOCK) / 8 * 8);
   }
   if (e == hipSuccess) e = hipMalloc((void **)&b->d_slots, b->max_slots * sizeof(BatchSlot));
