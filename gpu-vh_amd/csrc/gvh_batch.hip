@@ -38,8 +38,8 @@ namespace gvh {
 hipError_t launch_batch_stage(const BatchSlot *slots, const int *tile_slot, int begin, int end,
                               int eos, bool bulk, bool stage2, bool has_z, bool lock,
                               bool mixed, hipStream_t st);
-hipError_t launch_batch_record(BatchSlot *slots, const int *tile_slot, int ntiles, int grid,
-                               bool mixed, hipStream_t st);
+hipError_t launch_batch_record(BatchSlot *slots, const int *tile_slot, int ntiles, bool mixed,
+                               hipStream_t st);
 hipError_t launch_batch_check(BatchSlot *slots, int nslots, hipStream_t st);
 hipError_t launch_batch_advance(BatchSlot *slots, int nslots, hipStream_t st);
 }  // namespace gvh
@@ -139,7 +139,7 @@ struct gvh_batch {
   std::vector<double *> stage;
   std::vector<size_t> stage_cap;  // doubles
   size_t budget = 0, used = 0;
-  int max_slots = 0, fofreq = 10, autotune = 0, grid_cap = 1024;
+  int max_slots = 0, fofreq = 10, autotune = 0;
   BatchSlot *d_slots = nullptr;
   BatchSlot *h_slots = nullptr;  // pinned host mirror
   std::vector<Live> live;
@@ -265,11 +265,7 @@ int rebuild(gvh_batch *b) {
     return GVH_OK;
   };
   rc = capture([&]() {
-    // the record is a copy with few registers: up to 8 waves per SIMD of it
-    // keep enough loads in flight to stream HBM
-    const int rec_grid = std::max(8, std::min(b->ntiles_total, 8 * b->grid_cap));
-    hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total, rec_grid, mixed,
-                                       b->stream);
+    hipError_t e = launch_batch_record(b->d_slots, b->d_tiles, b->ntiles_total, mixed, b->stream);
     if (e == hipSuccess) e = launch_batch_check(b->d_slots, b->max_slots, b->stream);
     return e;
   }, b->graph, b->exec);
@@ -498,14 +494,6 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
   for (hipStream_t &q : b->search)
     if (e == hipSuccess) e = hipStreamCreate(&q);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->recorded, hipEventDisableTiming);
-  int ncu = 256;
-  if (e == hipSuccess) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // 16 waves per CU of BLOCK-thread blocks (the record kernel takes 8x)
-    b->grid_cap = std::max(8, ncu * 4 * (256 / GVH_BLOCK) / 8 * 8);
-  }
   if (e == hipSuccess) e = hipMalloc((void **)&b->d_slots, b->max_slots * sizeof(BatchSlot));
   if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_slots, b->max_slots * sizeof(BatchSlot));
   if (e == hipSuccess) {

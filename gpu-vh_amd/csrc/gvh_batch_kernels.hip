@@ -39,48 +39,46 @@ __global__ __launch_bounds__(BLOCK, stage_waves_per_simd<R>()) void kt_batch_ker
 // step's after an odd number of steps unless GVH_FO_CURRENT_FIELDS.
 template <typename PS>
 __global__ __launch_bounds__(BLOCK) void batch_record_kernel(BatchSlot *__restrict__ slots,
-                                                             const int *__restrict__ tile_slot,
-                                                             int ntiles) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int sl = tile_slot[t];
-    BatchSlot &S = slots[sl];
-    if (S.status != SLOT_RUN || S.steps >= S.max_steps) continue;
-    const CellPos c = cell_of_tile(S.A, (unsigned)(t - S.tile_first), (unsigned)S.ntiles);
-    const long long G = S.A.G;
-    const double *e = S.B.e[S.R.ie];
-    bool above = false;
-    if (c.valid) {
-      const unsigned s = c.s;
-      above = e[s] > S.e_fo;
-      if (S.ring) {
-        const bool stale = !(S.fo_flags & GVH_FO_CURRENT_FIELDS) && (S.steps & 1);
-        const double *u = S.B.u[stale ? S.R.iup : S.R.iu];
-        const PS *qP = reinterpret_cast<const PS *>(S.B.q[stale ? 1 - S.R.iq : S.R.iq] + 4 * G);
-        const long long nint = S.nint;
-        const long long cell =
-            (c.i - NG) + (long long)S.A.nx * ((c.j - NG) + (long long)S.A.ny * (c.k - NG));
-        const int n = S.steps + 1, nfo = n % S.fofreq;
-        double *ring = S.ring;
-        if (nfo == 0)
-          for (int v = 0; v < 10; ++v)
-            ring[(long long)v * nint + cell] = ring[((long long)S.fofreq * 10 + v) * nint + cell];
-        double *dst = ring + (long long)(nfo == 0 ? 1 : nfo + 1) * 10 * nint + cell;
-        dst[0 * nint] = u[1 * G + s];
-        dst[1 * nint] = u[2 * G + s];
-        dst[2 * nint] = u[3 * G + s];
-        dst[3 * nint] = e[s];
-        dst[4 * nint] = (double)qP[(GVH_PIXX - 4) * G + s];
-        dst[5 * nint] = (double)qP[(GVH_PIXY - 4) * G + s];
-        dst[6 * nint] = (double)qP[(GVH_PIXN - 4) * G + s];
-        dst[7 * nint] = (double)qP[(GVH_PIYY - 4) * G + s];
-        dst[8 * nint] = (double)qP[(GVH_PIYN - 4) * G + s];
-        dst[9 * nint] = S.bulk ? (double)qP[(GVH_PI - 4) * G + s] : 0.0;
-      }
+                                                             const int *__restrict__ tile_slot) {
+  const int t = blockIdx.x;
+  const int sl = tile_slot[t];
+  BatchSlot &S = slots[sl];
+  if (S.status != SLOT_RUN || S.steps >= S.max_steps) return;
+  const CellPos c = cell_of_tile(S.A, (unsigned)(t - S.tile_first), (unsigned)S.ntiles);
+  const long long G = S.A.G;
+  const double *e = S.B.e[S.R.ie];
+  bool above = false;
+  if (c.valid) {
+    const unsigned s = c.s;
+    above = e[s] > S.e_fo;
+    if (S.ring) {
+      const bool stale = !(S.fo_flags & GVH_FO_CURRENT_FIELDS) && (S.steps & 1);
+      const double *u = S.B.u[stale ? S.R.iup : S.R.iu];
+      const PS *qP = reinterpret_cast<const PS *>(S.B.q[stale ? 1 - S.R.iq : S.R.iq] + 4 * G);
+      const long long nint = S.nint;
+      const long long cell =
+          (c.i - NG) + (long long)S.A.nx * ((c.j - NG) + (long long)S.A.ny * (c.k - NG));
+      const int n = S.steps + 1, nfo = n % S.fofreq;
+      double *ring = S.ring;
+      if (nfo == 0)
+        for (int v = 0; v < 10; ++v)
+          ring[(long long)v * nint + cell] = ring[((long long)S.fofreq * 10 + v) * nint + cell];
+      double *dst = ring + (long long)(nfo == 0 ? 1 : nfo + 1) * 10 * nint + cell;
+      dst[0 * nint] = u[1 * G + s];
+      dst[1 * nint] = u[2 * G + s];
+      dst[2 * nint] = u[3 * G + s];
+      dst[3 * nint] = e[s];
+      dst[4 * nint] = (double)qP[(GVH_PIXX - 4) * G + s];
+      dst[5 * nint] = (double)qP[(GVH_PIXY - 4) * G + s];
+      dst[6 * nint] = (double)qP[(GVH_PIXN - 4) * G + s];
+      dst[7 * nint] = (double)qP[(GVH_PIYY - 4) * G + s];
+      dst[8 * nint] = (double)qP[(GVH_PIYN - 4) * G + s];
+      dst[9 * nint] = S.bulk ? (double)qP[(GVH_PI - 4) * G + s] : 0.0;
     }
-    // one atomic per wave
-    const unsigned long long m = __ballot(above);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&S.above, (unsigned long long)__popcll(m));
   }
+  // one atomic per wave
+  const unsigned long long m = __ballot(above);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&S.above, (unsigned long long)__popcll(m));
 }
 
 // Termination before the step (HydroPlugin.cpp:239, 396-413): an event ends
@@ -178,14 +176,15 @@ hipError_t launch_batch_stage(const BatchSlot *slots, const int *tile_slot, int 
   return hipGetLastError();
 }
 
-hipError_t launch_batch_record(BatchSlot *slots, const int *tile_slot, int ntiles, int grid,
-                               bool mixed, hipStream_t st) {
+// One block per tile of the tile map.
+hipError_t launch_batch_record(BatchSlot *slots, const int *tile_slot, int ntiles, bool mixed,
+                               hipStream_t st) {
   if (mixed)
-    hipLaunchKernelGGL(batch_record_kernel<float>, dim3(grid), dim3(BLOCK), 0, st, slots, tile_slot,
-                       ntiles);
+    hipLaunchKernelGGL(batch_record_kernel<float>, dim3(ntiles), dim3(BLOCK), 0, st, slots,
+                       tile_slot);
   else
-    hipLaunchKernelGGL(batch_record_kernel<double>, dim3(grid), dim3(BLOCK), 0, st, slots,
-                       tile_slot, ntiles);
+    hipLaunchKernelGGL(batch_record_kernel<double>, dim3(ntiles), dim3(BLOCK), 0, st, slots,
+                       tile_slot);
   return hipGetLastError();
 }
 
