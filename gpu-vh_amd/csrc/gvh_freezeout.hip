@@ -313,7 +313,11 @@ __global__ __launch_bounds__(64) void fo_cornelius_kernel(CellSource S, long lon
   __shared__ double cen[corn::MAXE][corn::D], nrm[corn::MAXE][corn::D];
   const int lane = threadIdx.x;
   const int ncorner = 1 << dim;
-  for (long long cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+  // one block (wave) per cell: cells differ much in cost, and the dispatcher
+  // balances single-cell blocks better than a loop over a fixed grid
+  {
+    const long long cell = blockIdx.x;
+    if (cell >= ncells) return;
     if (lane == 0) {
       W.dim = dim;
       W.v0 = v0;
@@ -682,7 +686,8 @@ int fo_find_elements(gvh_context *c, hipStream_t st) {
   HIP_TRY(hipGetLastError());
   CellSource S{W, f->cubes, nullptr, true};
   HIP_TRY(hipMemsetAsync(f->total + 1, 0, sizeof(unsigned), st));
-  const unsigned grid = (unsigned)std::min<long long>(ncubes, 8192);
+  if (ncubes > 0x7fffffffLL) return gvh_fail(GVH_EINVAL, "%lld surface cubes in one window", ncubes);
+  const unsigned grid = (unsigned)ncubes;
   const double dz = c->L.latticeSpacingRapidity;
   const double dt = c->L.latticeSpacingProperTime, dx = c->L.latticeSpacingX,
                dy = c->L.latticeSpacingY;
@@ -793,6 +798,7 @@ extern "C" int gvh_cornelius_cells(int dim, long long ncells, const double *corn
   if (ncells < 0 || (ncells > 0 && (!corners || !dx || !counts || !centroids || !normals)))
     return gvh_fail(GVH_EINVAL, "gvh_cornelius_cells: NULL argument");
   if (ncells == 0) return GVH_OK;
+  if (ncells > 0x7fffffffLL) return gvh_fail(GVH_EINVAL, "at most 2^31 - 1 cells per call");
   const int ncorner = 1 << dim;
   double *dv = nullptr, *del = nullptr;
   int *dcnt = nullptr;
@@ -807,7 +813,7 @@ extern "C" int gvh_cornelius_cells(int dim, long long ncells, const double *corn
     CellSource S{};
     S.window = false;
     S.corners = dv;
-    const unsigned grid = (unsigned)std::min<long long>(ncells, 8192);
+    const unsigned grid = (unsigned)ncells;
     fo_cornelius_kernel<<<grid, 64>>>(S, ncells, dim, dx[0], dx[1], dx[2], dim == 4 ? dx[3] : 0.0,
                                       v0, dcnt, del, dov);
     e = hipGetLastError();
