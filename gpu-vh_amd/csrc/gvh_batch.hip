@@ -11,10 +11,12 @@
 //   advance  buffer roles, step count                    (batch_advance_kernel)
 //
 // Launch groups are events of the same dimensionality and stage-kernel
-// arrangement.  The sequence is captured once into a HIP graph per batch
-// composition and replayed each step.  Between replays the host searches
-// the freeze-out windows that are due, retires finished events, and admits
-// queued ones while their HBM footprint (state, history ring, surface
+// arrangement.  The sequence is captured once per batch composition into
+// two HIP graphs (record and check; stages and advance) and replayed each
+// step.  The freeze-out windows due at a step are searched on other streams
+// once the first graph has run, beside the stages (the searches read only
+// the history ring).  Between steps the host retires finished events and
+// admits queued ones while their HBM footprint (state, history ring, surface
 // buffers) fits the budget: requests that arrive while others run join at
 // the next step.  An event's cells are cut into tiles exactly as a launch
 // of its own would cut them, so its results do not depend on the mix.

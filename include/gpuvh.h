@@ -314,8 +314,8 @@ int gvh_run_hydro(int argc, char **argv, const gvh_options *opts, const gvh_run_
  * reference, which runs one event per process (HydroPlugin.cpp:107-434):
  * each event here follows that run loop (nt + 1 steps at most, the
  * all-frozen stop of :396-413, freeze-out windows of :270-393) while all
- * running events advance together, one launch per kernel per step, from a
- * captured HIP graph.  Events share the batch's gvh_options (EoS, bulk,
+ * running events advance together, one launch per kernel per step, from two
+ * captured HIP graphs.  Events share the batch's gvh_options (EoS, bulk,
  * precision); each has its own lattice, time step and hydro parameters. */
 typedef struct gvh_batch gvh_batch;
 
