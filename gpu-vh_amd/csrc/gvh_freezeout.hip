@@ -16,7 +16,7 @@
 //     cubes in the reference's (it, ix, iy, ieta) order.
 //  3. fo_cornelius_kernel: one wave per surface cube, elements via the LDS
 //     restatement of cornelius (gvh_cornelius.h).
-//  4. fo_emit_kernel: one thread per surface cube writes its elements' rows
+//  4. fo_emit_kernel: one thread per (element, variable) writes the rows
 //     (position, normal, multilinearly interpolated hydro variables, T and P
 //     of the interpolated e), offsets from a scan of element counts.
 //
@@ -371,11 +371,10 @@ __global__ __launch_bounds__(64) void fo_cornelius_kernel(CellSource S, long lon
 struct EmitArgs {
   Window W;
   const unsigned *cubes;
-  const int *ecnt;
   const unsigned *eoff;
   const double *elem;
   double *rows;
-  long long ncubes;
+  long long ncubes, nrows;
   double t, t0, dt, dx, dy, dz;
   int n, fofreq, flags, eos;
   EosTable tab;
@@ -417,79 +416,97 @@ __device__ void eos_T_p(const EosTable &T, double e, double &temp, double &p) {
   p = eos_p<EOS>(T, e);
 }
 
-__global__ __launch_bounds__(128) void fo_emit_kernel(EmitArgs A) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= A.ncubes) return;
+// One thread per (element, ring variable): the element's cube is the last one
+// whose row offset is <= the element's row (eoff is the exclusive scan of the
+// element counts), the variable's 2^dim corners are gathered at once and
+// interpolated.  Variable 0 also writes the position and normal, variable 3
+// (e) also T and P.  Same arithmetic per value as one thread per element.
+// Blocks go to the 8 XCDs in turn; the grid is a multiple of 8 and block b
+// takes the range (b mod 8) nb/8 + b/8, so each XCD walks a contiguous run of
+// the (it, ix, iy, ieta)-ordered elements and neighbouring cube columns,
+// which share corner cache lines, meet in the same L2.
+template <int DIM>
+__global__ __launch_bounds__(256) void fo_emit_kernel(EmitArgs A) {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  const long long g = (long long)lb * blockDim.x + threadIdx.x;
+  if (g >= A.nrows * NVAR) return;
+  const long long j = g / NVAR;
+  const int v = (int)(g - j * NVAR);
+  long long lo = 0, hi = A.ncubes - 1;
+  while (lo < hi) {
+    const long long mid = (lo + hi + 1) >> 1;
+    if ((long long)A.eoff[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  const long long idx = lo;
+  const int i = (int)(j - (long long)A.eoff[idx]);
   const Window &W = A.W;
   int itl, ix, iy, iz;
   decode_cube(W, A.cubes[idx], itl, ix, iy, iz);
   const int it = W.start + itl;
-  const int n = A.ecnt[idx];
-  const unsigned base = A.eoff[idx];
-  // cell tau (HydroPlugin.cpp:304-305)
-  double cell_tau;
-  if (A.n <= A.fofreq || (A.flags & GVH_FO_EXACT_TAU))
-    cell_tau = A.t0 + ((double)(A.n - A.fofreq + (it - 1))) * A.dt;
-  else
-    cell_tau = A.t0 + ((double)(A.n - A.fofreq + it)) * A.dt;
-  const double cell_x = (double)ix * A.dx - (((double)(W.nx - 1)) / 2.0 * A.dx);
-  const double cell_y = (double)iy * A.dy - (((double)(W.ny - 1)) / 2.0 * A.dy);
-  const double cell_z = (double)iz * A.dz - (((double)(W.nz - 1)) / 2.0 * A.dz);
-  const int nc = 1 << W.dim;
-  for (int i = 0; i < n; ++i) {
-    const double *el = A.elem + (idx * corn::MAXE + i) * ELEM;
-    double c[4], m[4], fr[4];
-    if (W.dim == 4) {
-      for (int k = 0; k < 4; ++k) {
-        c[k] = el[k];
-        m[k] = el[4 + k];
-      }
-    } else {
-      for (int k = 0; k < 3; ++k) {
-        c[k] = el[1 + k];
-        m[k] = el[5 + k];
-      }
-      c[3] = 0;
-      m[3] = 0;
+  const double *el = A.elem + (idx * corn::MAXE + i) * ELEM;
+  double c[4], m[4], fr[4];
+  if constexpr (DIM == 4) {
+    for (int k = 0; k < 4; ++k) {
+      c[k] = el[k];
+      m[k] = el[4 + k];
     }
-    fr[0] = c[0] / A.dt;
-    fr[1] = c[1] / A.dx;
-    fr[2] = c[2] / A.dy;
-    fr[3] = W.dim == 4 ? c[3] / A.dz : 0.0;
-    double *row = A.rows + (size_t)(base + i) * NCOL;
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      c[k] = el[1 + k];
+      m[k] = el[5 + k];
+    }
+    c[3] = 0;
+    m[3] = 0;
+  }
+  fr[0] = c[0] / A.dt;
+  fr[1] = c[1] / A.dx;
+  fr[2] = c[2] / A.dy;
+  fr[3] = DIM == 4 ? c[3] / A.dz : 0.0;
+  double *row = A.rows + (size_t)j * NCOL;
+  constexpr int nc = 1 << DIM;
+  double a[nc];
+#pragma unroll
+  for (int k = 0; k < nc; ++k) {
+    const int ta = DIM == 4 ? k >> 3 : k >> 2;
+    const int xb = DIM == 4 ? (k >> 2) & 1 : (k >> 1) & 1;
+    const int yc = DIM == 4 ? (k >> 1) & 1 : k & 1;
+    const int zd = DIM == 4 ? k & 1 : 0;
+    const double *slot = W.ring + ((long long)W.sl.phys[it + ta] * NVAR + v) * W.nint;
+    a[k] = slot[W.cell(ix + xb, iy + yc, iz + zd)];
+  }
+  double val;
+  if constexpr (DIM == 4) val = interp4(fr[0], fr[1], fr[2], fr[3], a);
+  else val = interp3(fr[0], fr[1], fr[2], a);
+  if (v == 0) {
+    // cell tau (HydroPlugin.cpp:304-305)
+    double cell_tau;
+    if (A.n <= A.fofreq || (A.flags & GVH_FO_EXACT_TAU))
+      cell_tau = A.t0 + ((double)(A.n - A.fofreq + (it - 1))) * A.dt;
+    else
+      cell_tau = A.t0 + ((double)(A.n - A.fofreq + it)) * A.dt;
+    const double cell_x = (double)ix * A.dx - (((double)(W.nx - 1)) / 2.0 * A.dx);
+    const double cell_y = (double)iy * A.dy - (((double)(W.ny - 1)) / 2.0 * A.dy);
+    const double cell_z = (double)iz * A.dz - (((double)(W.nz - 1)) / 2.0 * A.dz);
     row[0] = c[0] + cell_tau;
     row[1] = c[1] + cell_x;
     row[2] = c[2] + cell_y;
-    row[3] = W.dim == 4 ? c[3] + cell_z : cell_z;
+    row[3] = DIM == 4 ? c[3] + cell_z : cell_z;
     const double scale = (A.flags & GVH_FO_NORMAL_TAU) ? row[0] : A.t;
     row[4] = scale * m[0];
     row[5] = scale * m[1];
     row[6] = scale * m[2];
-    row[7] = W.dim == 4 ? scale * m[3] : 0.0;
-    double val[NVAR];
-    for (int v = 0; v < NVAR; ++v) {
-      double a[16];
-      for (int k = 0; k < nc; ++k) {
-        const int ta = W.dim == 4 ? k >> 3 : k >> 2;
-        const int xb = W.dim == 4 ? (k >> 2) & 1 : (k >> 1) & 1;
-        const int yc = W.dim == 4 ? (k >> 1) & 1 : k & 1;
-        const int zd = W.dim == 4 ? k & 1 : 0;
-        const double *slot = W.ring + ((long long)W.sl.phys[it + ta] * NVAR + v) * W.nint;
-        a[k] = slot[W.cell(ix + xb, iy + yc, iz + zd)];
-      }
-      val[v] = W.dim == 4 ? interp4(fr[0], fr[1], fr[2], fr[3], a) : interp3(fr[0], fr[1], fr[2], a);
-    }
-    row[8] = val[0];
-    row[9] = val[1];
-    row[10] = val[2];
-    row[11] = val[3];
+    row[7] = DIM == 4 ? scale * m[3] : 0.0;
+  }
+  row[v < 4 ? 8 + v : 10 + v] = val;
+  if (v == 3) {
     double T, P;
-    if (A.eos == EOS_CONFORMAL) eos_T_p<EOS_CONFORMAL>(A.tab, val[3], T, P);
-    else if (A.eos == EOS_TABLE) eos_T_p<EOS_TABLE>(A.tab, val[3], T, P);
-    else eos_T_p<EOS_WB>(A.tab, val[3], T, P);
+    if (A.eos == EOS_CONFORMAL) eos_T_p<EOS_CONFORMAL>(A.tab, val, T, P);
+    else if (A.eos == EOS_TABLE) eos_T_p<EOS_TABLE>(A.tab, val, T, P);
+    else eos_T_p<EOS_WB>(A.tab, val, T, P);
     row[12] = T;
     row[13] = P;
-    for (int v = 4; v < NVAR; ++v) row[10 + v] = val[v];
   }
 }
 
@@ -715,15 +732,17 @@ int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements)
   const long long nrows = f->htot[0];
   int rc;
   if ((rc = ensure_out_rows(f, (size_t)nrows))) return rc;
+  if (nrows * NVAR > 0xfffff000LL)
+    return gvh_fail(GVH_EINVAL, "%lld surface elements in one window", nrows);
   const int start = f->n_pending <= f->fofreq ? 2 : 0;
   EmitArgs A{};
   A.W = make_window(c, f, start);
   A.cubes = f->cubes;
-  A.ecnt = f->ecnt;
   A.eoff = f->eoff;
   A.elem = f->elem;
   A.rows = f->rows;
   A.ncubes = f->ncubes;
+  A.nrows = nrows;
   A.t = t;
   A.t0 = c->Hp.initialProperTimePoint;
   A.dt = c->L.latticeSpacingProperTime;
@@ -735,7 +754,9 @@ int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements)
   A.flags = f->flags;
   A.eos = c->eos.kind;
   A.tab = c->tab;
-  fo_emit_kernel<<<grid_for(f->ncubes, 128), 128, 0, st>>>(A);
+  const unsigned egrid = (grid_for(nrows * NVAR, 256) + 7u) / 8u * 8u;  // multiple of 8 (XCD order)
+  if (nrows > 0 && f->dim == 4) fo_emit_kernel<4><<<egrid, 256, 0, st>>>(A);
+  else if (nrows > 0) fo_emit_kernel<3><<<egrid, 256, 0, st>>>(A);
   HIP_TRY(hipGetLastError());
   f->nrows = nrows;
   *nelements = nrows;
