@@ -58,118 +58,154 @@ struct Work {
   short hntet[MAXE];
   int cfirst[8], cnl[8], camb[8], cnp[8];
   int np, nh, nlines, overflow;
+  double tet[MAXL][D + 1];           // per-tetrahedron terms of one polyhedron
   double v[16];
   double dx[D];
   double v0;
   int dim;
 };
 
+// Component i of a 4-vector held in registers, by selects (a run-time index
+// into a private array would put the array in scratch memory).
+__device__ __forceinline__ double pick(const double *v, int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
 // Cube q of a cell: the axis held constant, its value and the three free
-// axes in increasing order.  A 3-D cell is the cube with axis 0 held at 0.
-__device__ __forceinline__ void cube_axes(const Work &W, int q, int &cst, double &cv, int x[3]) {
+// axes x0 < x1 < x2.  A 3-D cell is the cube with axis 0 held at 0.
+__device__ __forceinline__ void cube_axes(const Work &W, int q, int &cst, double &cv, int &x0,
+                                          int &x1, int &x2) {
   if (W.dim == 4) {
     cst = q >> 1;
     cv = (q & 1) * W.dx[cst];
-    int k = 0;
-    for (int i = 0; i < D; ++i)
-      if (i != cst) x[k++] = i;
+    x0 = cst == 0 ? 1 : 0;
+    x1 = cst <= 1 ? 2 : 1;
+    x2 = cst <= 2 ? 3 : 2;
   } else {
     cst = 0;
     cv = 0.0;
-    x[0] = 1;
-    x[1] = 2;
-    x[2] = 3;
+    x0 = 1;
+    x1 = 2;
+    x2 = 3;
   }
 }
 
-// Corner (a, b, c) of cube q over its free axes.
+// Corner (a, b, c) of cube q over its free axes: axis k takes q's side j if
+// it is the held axis, else the next of a, b, c.
 __device__ __forceinline__ double cube_value(const Work &W, int q, int a, int b, int c) {
   if (W.dim == 3) return W.v[4 * a + 2 * b + c];
   const int cst = q >> 1, j = q & 1;
-  const int f[3] = {a, b, c};
-  int idx[4], m = 0;
-  for (int k = 0; k < 4; ++k) idx[k] = (k == cst) ? j : f[m++];
-  return W.v[8 * idx[0] + 4 * idx[1] + 2 * idx[2] + idx[3]];
+  const int i0 = cst == 0 ? j : a;
+  const int i1 = cst == 1 ? j : (cst == 0 ? a : b);
+  const int i2 = cst == 2 ? j : (cst <= 1 ? b : c);
+  const int i3 = cst == 3 ? j : c;
+  return W.v[8 * i0 + 4 * i1 + 2 * i2 + i3];
 }
 
-// Lines of one square: values sq[a][b] at (a h1, b h2) along free axes
-// x1 < x2; constant axes c0, c1 at values cv0, cv1.  Returns 0, 1 or 2.
-__device__ int square_lines(const double sq[2][2], int c0, int c1, double cv0, double cv1, int x1,
-                            int x2, const double *dx, double v0, LineRec res[2]) {
+// Cut points of one square with values sq[a][b] at (a h1, b h2) along its
+// free axes x1 < x2, in the coordinates of those axes: up to four, in the
+// edge order below, and the outside point of each of its (nc / 2) lines.
+// Every index is fixed at compile time, so the result stays in registers.
+struct SquareCut {
+  double cut[4][2];
+  double opt[2][2];
+  int nl;
+};
+
+__device__ __forceinline__ SquareCut square_cut(const double sq[2][2], double h1, double h2,
+                                                double v0) {
+  SquareCut r;
+  r.nl = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.cut[k][0] = r.cut[k][1] = 0.0;
+  r.opt[0][0] = r.opt[0][1] = r.opt[1][0] = r.opt[1][1] = 0.0;
   int above = 0;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) above += sq[a][b] >= v0;
-  if (above == 0 || above == 4) return 0;
-  const double h1 = dx[x1], h2 = dx[x2];
-  double cut[4][2];
+  if (above == 0 || above == 4) return r;
+  // edges from corner value a to corner value b: along x1 at x2 = 0, along
+  // x2 at x1 = 0, along x2 at x1 = h1, along x1 at x2 = h2
+  const double ea[4] = {sq[0][0], sq[0][0], sq[1][0], sq[0][1]};
+  const double eb[4] = {sq[1][0], sq[0][1], sq[1][1], sq[1][1]};
+  const double fix[4] = {0.0, 0.0, h1, h2};
+  const double hh[4] = {h1, h2, h2, h1};
+  const bool along1[4] = {true, false, false, true};
   int nc = 0;
-  // edge from corner value a to corner value b, along x1 (along1) or x2, at
-  // coordinate `fix` of the other free axis
-  auto edge = [&](double a, double b, double fix, bool along1, double h) {
-    double s;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double a = ea[e], b = eb[e], h = hh[e];
+    double s = 0.0;
+    bool cut = true;
     if ((a - v0) * (b - v0) < 0) s = (a - v0) / (a - b) * h;
     else if (a == v0 && b < v0) s = 1e-9 * h;
     else if (b == v0 && a < v0) s = (1.0 - 1e-9) * h;
-    else return;
-    cut[nc][0] = along1 ? s : fix;
-    cut[nc][1] = along1 ? fix : s;
-    ++nc;
-  };
-  edge(sq[0][0], sq[1][0], 0.0, true, h1);
-  edge(sq[0][0], sq[0][1], 0.0, false, h2);
-  edge(sq[1][0], sq[1][1], h1, false, h2);
-  edge(sq[0][1], sq[1][1], h2, true, h1);
-  double opt[2][2] = {{0, 0}, {0, 0}};
+    else cut = false;
+    if (cut) {
+      const double p0 = along1[e] ? s : fix[e], p1 = along1[e] ? fix[e] : s;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k == nc) {
+          r.cut[k][0] = p0;
+          r.cut[k][1] = p1;
+        }
+      ++nc;
+    }
+  }
   if (nc == 4) {
     double mid = 0;
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b) mid += 0.25 * sq[a][b];
     if ((sq[0][0] < v0 && mid < v0) || (sq[0][0] > v0 && mid > v0))
       for (int k = 0; k < 2; ++k) {
-        const double t = cut[1][k];
-        cut[1][k] = cut[2][k];
-        cut[2][k] = t;
+        const double t = r.cut[1][k];
+        r.cut[1][k] = r.cut[2][k];
+        r.cut[2][k] = t;
       }
     if (mid - v0 < 0) {
-      opt[0][0] = opt[1][0] = 0.5 * h1;
-      opt[0][1] = opt[1][1] = 0.5 * h2;
+      r.opt[0][0] = r.opt[1][0] = 0.5 * h1;
+      r.opt[0][1] = r.opt[1][1] = 0.5 * h2;
     } else if (sq[0][0] - v0 < 0) {
-      opt[1][0] = h1;
-      opt[1][1] = h2;
+      r.opt[1][0] = h1;
+      r.opt[1][1] = h2;
     } else {
-      opt[0][0] = h1;
-      opt[1][1] = h2;
+      r.opt[0][0] = h1;
+      r.opt[1][1] = h2;
     }
   } else if (nc == 2) {
     int nout = 0;
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b)
         if (sq[a][b] < v0) {
-          opt[0][0] += a * h1;
-          opt[0][1] += b * h2;
+          r.opt[0][0] += a * h1;
+          r.opt[0][1] += b * h2;
           ++nout;
         }
     if (nout > 0) {
-      opt[0][0] /= nout;
-      opt[0][1] /= nout;
+      r.opt[0][0] /= nout;
+      r.opt[0][1] /= nout;
     }
   }
-  int nl = 0;
-  for (int k = 0; k + 1 < nc; k += 2) {
-    LineRec &l = res[nl++];
-    for (int e = 0; e < 2; ++e) {
-      double *p = e ? l.e : l.s;
-      p[x1] = cut[k + e][0];
-      p[x2] = cut[k + e][1];
-      p[c0] = cv0;
-      p[c1] = cv1;
-    }
-    l.o[x1] = opt[k / 2][0];
-    l.o[x2] = opt[k / 2][1];
-    l.o[c0] = cv0;
-    l.o[c1] = cv1;
-  }
-  return nl;
+  r.nl = nc >> 1;
+  return r;
+}
+
+// Line K of a square into l: free axes x1, x2 take the cut coordinates,
+// constant axes c0, c1 the values cv0, cv1.
+template <int K>
+__device__ __forceinline__ void store_line(LineRec &l, const SquareCut &r, int c0, int c1,
+                                           double cv0, double cv1, int x1, int x2) {
+  l.s[x1] = r.cut[2 * K][0];
+  l.s[x2] = r.cut[2 * K][1];
+  l.s[c0] = cv0;
+  l.s[c1] = cv1;
+  l.e[x1] = r.cut[2 * K + 1][0];
+  l.e[x2] = r.cut[2 * K + 1][1];
+  l.e[c0] = cv0;
+  l.e[c1] = cv1;
+  l.o[x1] = r.opt[K][0];
+  l.o[x2] = r.opt[K][1];
+  l.o[c0] = cv0;
+  l.o[c1] = cv1;
 }
 
 __device__ __forceinline__ bool same_point_lt(const double *a, const double *b) {
@@ -212,10 +248,9 @@ __device__ __forceinline__ const LineRec &pline(const Work &W, int g, int k) {
 // Centroid (and, for 3-D cells, normal) of polygon g.
 __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
   const int n = W.plen[g];
-  int cst, x[3];
+  int cst, x1, x2, x3;
   double cv;
-  cube_axes(W, W.pcube[g], cst, cv, x);
-  const int x1 = x[0], x2 = x[1], x3 = x[2];
+  cube_axes(W, W.pcube[g], cst, cv, x1, x2, x3);
   double mean[D] = {0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
     const LineRec &l = pline(W, g, i);
@@ -235,9 +270,9 @@ __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
         a[j] = l.s[j] - mean[j];
         b[j] = l.e[j] - mean[j];
       }
-      const double u1 = a[x2] * b[x3] - a[x3] * b[x2];
-      const double u2 = a[x1] * b[x3] - a[x3] * b[x1];
-      const double u3 = a[x2] * b[x1] - a[x1] * b[x2];
+      const double u1 = pick(a, x2) * pick(b, x3) - pick(a, x3) * pick(b, x2);
+      const double u2 = pick(a, x1) * pick(b, x3) - pick(a, x3) * pick(b, x1);
+      const double u3 = pick(a, x2) * pick(b, x1) - pick(a, x1) * pick(b, x2);
       const double A = 0.5 * sqrt(u1 * u1 + u2 * u2 + u3 * u3);
       for (int j = 0; j < D; ++j) up[j] += cm[j] * A;
       down += A;
@@ -254,10 +289,10 @@ __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
       a[j] = l.s[j] - cen[j];
       b[j] = l.e[j] - cen[j];
     }
-    nrm[x1] = 0.5 * (a[x2] * b[x3] - a[x3] * b[x2]);
-    nrm[x2] = -0.5 * (a[x1] * b[x3] - a[x3] * b[x1]);
-    nrm[x3] = 0.5 * (a[x1] * b[x2] - a[x2] * b[x1]);
-    nrm[cst] = 0;
+    const double n1 = 0.5 * (pick(a, x2) * pick(b, x3) - pick(a, x3) * pick(b, x2));
+    const double n2 = -0.5 * (pick(a, x1) * pick(b, x3) - pick(a, x3) * pick(b, x1));
+    const double n3 = 0.5 * (pick(a, x1) * pick(b, x2) - pick(a, x2) * pick(b, x1));
+    for (int j = 0; j < D; ++j) nrm[j] = j == x1 ? n1 : (j == x2 ? n2 : (j == x3 ? n3 : 0.0));
     for (int j = 0; j < D; ++j) o[j] = l.o[j] - cen[j];
     orient(nrm, o);
     for (int j = 0; j < D; ++j) nsum[j] += nrm[j];
@@ -265,9 +300,29 @@ __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
   for (int j = 0; j < D; ++j) W.pnrm[g][j] = nsum[j];
 }
 
-// Centroid and normal of polyhedron h.
-__device__ void polyhedron_geometry(const Work &W, int h, double *cen, double *nrm) {
-  const int first = W.hfirst[h], np = W.hlen[h];
+// Polygon g and line j of tetrahedron t of a polyhedron whose np polygons
+// are listed from W.hlist[first]: its lines polygon by polygon.
+__device__ __forceinline__ void tet_line(const Work &W, int first, int np, int t, int &g, int &j) {
+  g = W.hlist[first];
+  j = 0;
+  for (int i = 0; i < np; ++i) {
+    const int gi = W.hlist[first + i];
+    const int n = W.plen[gi];
+    if (t < n) {
+      g = gi;
+      j = t;
+      return;
+    }
+    t -= n;
+  }
+}
+
+// Centroid and normal of polyhedron h, called by all lanes of the wave.  The
+// tetrahedra (one per line of each polygon) are spread over the lanes; their
+// terms go to W.tet and every lane sums them in tetrahedron order, the order
+// of the reference's serial loops, so the sums round the same way.
+__device__ void polyhedron_geometry(Work &W, int h, int lane, double *cen_out, double *nrm_out) {
+  const int first = W.hfirst[h], np = W.hlen[h], ntet = W.hntet[h];
   double mean[D] = {0, 0, 0, 0};
   for (int i = 0; i < np; ++i) {
     const int g = W.hlist[first + i];
@@ -276,47 +331,61 @@ __device__ void polyhedron_geometry(const Work &W, int h, double *cen, double *n
       for (int k = 0; k < D; ++k) mean[k] += l.s[k] + l.e[k];
     }
   }
-  for (int k = 0; k < D; ++k) mean[k] = mean[k] / double(2.0 * W.hntet[h]);
+  for (int k = 0; k < D; ++k) mean[k] = mean[k] / double(2.0 * ntet);
+  for (int t = lane; t < ntet; t += 64) {
+    int g, j;
+    tet_line(W, first, np, t, g, j);
+    const LineRec &l = pline(W, g, j);
+    const double *pc = W.pcen[g];
+    double a[D], b[D], c[D], n[D], cm[D];
+    for (int k = 0; k < D; ++k) {
+      cm[k] = (l.s[k] + l.e[k] + pc[k] + mean[k]) / 4.0;
+      a[k] = l.s[k] - mean[k];
+      b[k] = l.e[k] - mean[k];
+      c[k] = pc[k] - mean[k];
+    }
+    tetra_normal(a, b, c, n);
+    double V = 0;
+    for (int k = 0; k < D; ++k) V += n[k] * n[k];
+    V = sqrt(V);
+    for (int k = 0; k < D; ++k) W.tet[t][k] = cm[k] * V;
+    W.tet[t][D] = V;
+  }
+  __syncthreads();
   double up[D] = {0, 0, 0, 0}, down = 0;
-  for (int i = 0; i < np; ++i) {
-    const int g = W.hlist[first + i];
-    const double *pc = W.pcen[g];
-    for (int j = 0; j < W.plen[g]; ++j) {
-      const LineRec &l = pline(W, g, j);
-      double a[D], b[D], c[D], n[D], cm[D];
-      for (int k = 0; k < D; ++k) {
-        cm[k] = (l.s[k] + l.e[k] + pc[k] + mean[k]) / 4.0;
-        a[k] = l.s[k] - mean[k];
-        b[k] = l.e[k] - mean[k];
-        c[k] = pc[k] - mean[k];
-      }
-      tetra_normal(a, b, c, n);
-      double V = 0;
-      for (int k = 0; k < D; ++k) V += n[k] * n[k];
-      V = sqrt(V);
-      for (int k = 0; k < D; ++k) up[k] += cm[k] * V;
-      down += V;
-    }
+  for (int t = 0; t < ntet; ++t) {
+    for (int k = 0; k < D; ++k) up[k] += W.tet[t][k];
+    down += W.tet[t][D];
   }
+  double cen[D];
   for (int k = 0; k < D; ++k) cen[k] = up[k] / down;
-  for (int k = 0; k < D; ++k) nrm[k] = 0;
-  for (int i = 0; i < np; ++i) {
-    const int g = W.hlist[first + i];
+  __syncthreads();
+  for (int t = lane; t < ntet; t += 64) {
+    int g, j;
+    tet_line(W, first, np, t, g, j);
+    const LineRec &l = pline(W, g, j);
     const double *pc = W.pcen[g];
-    for (int j = 0; j < W.plen[g]; ++j) {
-      const LineRec &l = pline(W, g, j);
-      double a[D], b[D], c[D], n[D], o[D];
-      for (int k = 0; k < D; ++k) {
-        a[k] = l.s[k] - cen[k];
-        b[k] = l.e[k] - cen[k];
-        c[k] = pc[k] - cen[k];
-        o[k] = l.o[k] - cen[k];
-      }
-      tetra_normal(a, b, c, n);
-      orient(n, o);
-      for (int k = 0; k < D; ++k) nrm[k] += n[k];
+    double a[D], b[D], c[D], n[D], o[D];
+    for (int k = 0; k < D; ++k) {
+      a[k] = l.s[k] - cen[k];
+      b[k] = l.e[k] - cen[k];
+      c[k] = pc[k] - cen[k];
+      o[k] = l.o[k] - cen[k];
     }
+    tetra_normal(a, b, c, n);
+    orient(n, o);
+    for (int k = 0; k < D; ++k) W.tet[t][k] = n[k];
   }
+  __syncthreads();
+  double nrm[D] = {0, 0, 0, 0};
+  for (int t = 0; t < ntet; ++t)
+    for (int k = 0; k < D; ++k) nrm[k] += W.tet[t][k];
+  if (lane == 0)
+    for (int k = 0; k < D; ++k) {
+      cen_out[k] = cen[k];
+      nrm_out[k] = nrm[k];
+    }
+  __syncthreads();
 }
 
 // Lines joined into polyhedra when the start or the end of a line of the
@@ -347,27 +416,36 @@ __device__ int cell_elements(Work &W, double (*cen)[D], double (*nrm)[D]) {
     if (above == 0 || above == nc) return 0;
   }
   // --- 1: squares -> lines
-  LineRec tmp[2];
-  int cnt = 0;
+  SquareCut sc;
+  sc.nl = 0;
+  int c0 = 0, c1 = 0, x1 = 0, x2 = 0;
+  double cv0 = 0.0, cv1 = 0.0;
   const int q = lane / 6, r = lane % 6, f = r >> 1, jj = r & 1;
   if (lane < 6 * ncube) {
-    int cst, x[3];
+    // square (f, jj) of cube q: free axis f held at side jj
+    int cst, a0, a1, a2;
     double cv;
-    cube_axes(W, q, cst, cv, x);
+    cube_axes(W, q, cst, cv, a0, a1, a2);
     double sq[2][2];
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b)
         sq[a][b] = f == 0 ? cube_value(W, q, jj, a, b)
                           : (f == 1 ? cube_value(W, q, a, jj, b) : cube_value(W, q, a, b, jj));
-    int fr[2], m = 0;
-    for (int k = 0; k < 3; ++k)
-      if (k != f) fr[m++] = x[k];
-    cnt = square_lines(sq, cst, x[f], cv, jj * W.dx[x[f]], fr[0], fr[1], W.dx, W.v0, tmp);
+    const int xf = f == 0 ? a0 : (f == 1 ? a1 : a2);
+    x1 = f == 0 ? a1 : a0;
+    x2 = f == 2 ? a1 : a2;
+    c0 = cst;
+    c1 = xf;
+    cv0 = cv;
+    cv1 = jj * W.dx[xf];
+    sc = square_cut(sq, W.dx[x1], W.dx[x2], W.v0);
   }
+  const int cnt = sc.nl;
   const unsigned long long b1 = __ballot(cnt >= 1), b2 = __ballot(cnt == 2);
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int pos = __popcll(b1 & lt) + __popcll(b2 & lt);
-  for (int i = 0; i < cnt; ++i) W.L[pos + i] = tmp[i];
+  if (cnt >= 1) store_line<0>(W.L[pos], sc, c0, c1, cv0, cv1, x1, x2);
+  if (cnt == 2) store_line<1>(W.L[pos + 1], sc, c0, c1, cv0, cv1, x1, x2);
   if (lane < ncube) {
     const unsigned long long below = (1ull << (6 * lane)) - 1ull;
     const unsigned long long mine = 0x3full << (6 * lane);
@@ -516,7 +594,7 @@ __device__ int cell_elements(Work &W, double (*cen)[D], double (*nrm)[D]) {
   __syncthreads();
   // --- 5: polyhedron centroids and normals
   const int nh = W.nh;
-  if (lane < nh) polyhedron_geometry(W, lane, cen[lane], nrm[lane]);
+  for (int h = 0; h < nh; ++h) polyhedron_geometry(W, h, lane, cen[h], nrm[h]);
   return nh;
 }
 
