@@ -318,41 +318,56 @@ __device__ __forceinline__ void tet_line(const Work &W, int first, int np, int t
 }
 
 // Centroid and normal of polyhedron h, called by all lanes of the wave.  The
-// tetrahedra (one per line of each polygon) are spread over the lanes; their
-// terms go to W.tet and every lane sums them in tetrahedron order, the order
-// of the reference's serial loops, so the sums round the same way.
+// tetrahedra (one per line of each polygon, at most 96) are spread over the
+// lanes, lane l taking tetrahedra l and l + 64; their terms go to W.tet and
+// every lane sums the rows in tetrahedron order, the order of the
+// reference's serial loops, so the sums round the same way.
 __device__ void polyhedron_geometry(Work &W, int h, int lane, double *cen_out, double *nrm_out) {
   const int first = W.hfirst[h], np = W.hlen[h], ntet = W.hntet[h];
+  static_assert(MAXL <= 128, "two tetrahedra per lane");
+  int g[2] = {0, 0}, j[2] = {0, 0};
+  const LineRec *ln[2] = {nullptr, nullptr};
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (lane + 64 * r < ntet) {
+      tet_line(W, first, np, lane + 64 * r, g[r], j[r]);
+      ln[r] = &pline(W, g[r], j[r]);
+    }
+  // mean of the line end points
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (ln[r])
+      for (int k = 0; k < D; ++k) W.tet[lane + 64 * r][k] = ln[r]->s[k] + ln[r]->e[k];
+  __syncthreads();
   double mean[D] = {0, 0, 0, 0};
-  for (int i = 0; i < np; ++i) {
-    const int g = W.hlist[first + i];
-    for (int j = 0; j < W.plen[g]; ++j) {
-      const LineRec &l = pline(W, g, j);
-      for (int k = 0; k < D; ++k) mean[k] += l.s[k] + l.e[k];
-    }
-  }
+#pragma unroll 4
+  for (int t = 0; t < ntet; ++t)
+    for (int k = 0; k < D; ++k) mean[k] += W.tet[t][k];
   for (int k = 0; k < D; ++k) mean[k] = mean[k] / double(2.0 * ntet);
-  for (int t = lane; t < ntet; t += 64) {
-    int g, j;
-    tet_line(W, first, np, t, g, j);
-    const LineRec &l = pline(W, g, j);
-    const double *pc = W.pcen[g];
-    double a[D], b[D], c[D], n[D], cm[D];
-    for (int k = 0; k < D; ++k) {
-      cm[k] = (l.s[k] + l.e[k] + pc[k] + mean[k]) / 4.0;
-      a[k] = l.s[k] - mean[k];
-      b[k] = l.e[k] - mean[k];
-      c[k] = pc[k] - mean[k];
+  __syncthreads();
+  // volume-weighted centroid
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (ln[r]) {
+      const LineRec &l = *ln[r];
+      const double *pc = W.pcen[g[r]];
+      double a[D], b[D], c[D], n[D], cm[D];
+      for (int k = 0; k < D; ++k) {
+        cm[k] = (l.s[k] + l.e[k] + pc[k] + mean[k]) / 4.0;
+        a[k] = l.s[k] - mean[k];
+        b[k] = l.e[k] - mean[k];
+        c[k] = pc[k] - mean[k];
+      }
+      tetra_normal(a, b, c, n);
+      double V = 0;
+      for (int k = 0; k < D; ++k) V += n[k] * n[k];
+      V = sqrt(V);
+      for (int k = 0; k < D; ++k) W.tet[lane + 64 * r][k] = cm[k] * V;
+      W.tet[lane + 64 * r][D] = V;
     }
-    tetra_normal(a, b, c, n);
-    double V = 0;
-    for (int k = 0; k < D; ++k) V += n[k] * n[k];
-    V = sqrt(V);
-    for (int k = 0; k < D; ++k) W.tet[t][k] = cm[k] * V;
-    W.tet[t][D] = V;
-  }
   __syncthreads();
   double up[D] = {0, 0, 0, 0}, down = 0;
+#pragma unroll 4
   for (int t = 0; t < ntet; ++t) {
     for (int k = 0; k < D; ++k) up[k] += W.tet[t][k];
     down += W.tet[t][D];
@@ -360,24 +375,26 @@ __device__ void polyhedron_geometry(Work &W, int h, int lane, double *cen_out, d
   double cen[D];
   for (int k = 0; k < D; ++k) cen[k] = up[k] / down;
   __syncthreads();
-  for (int t = lane; t < ntet; t += 64) {
-    int g, j;
-    tet_line(W, first, np, t, g, j);
-    const LineRec &l = pline(W, g, j);
-    const double *pc = W.pcen[g];
-    double a[D], b[D], c[D], n[D], o[D];
-    for (int k = 0; k < D; ++k) {
-      a[k] = l.s[k] - cen[k];
-      b[k] = l.e[k] - cen[k];
-      c[k] = pc[k] - cen[k];
-      o[k] = l.o[k] - cen[k];
+  // normal: the oriented tetrahedron normals about the centroid
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (ln[r]) {
+      const LineRec &l = *ln[r];
+      const double *pc = W.pcen[g[r]];
+      double a[D], b[D], c[D], n[D], o[D];
+      for (int k = 0; k < D; ++k) {
+        a[k] = l.s[k] - cen[k];
+        b[k] = l.e[k] - cen[k];
+        c[k] = pc[k] - cen[k];
+        o[k] = l.o[k] - cen[k];
+      }
+      tetra_normal(a, b, c, n);
+      orient(n, o);
+      for (int k = 0; k < D; ++k) W.tet[lane + 64 * r][k] = n[k];
     }
-    tetra_normal(a, b, c, n);
-    orient(n, o);
-    for (int k = 0; k < D; ++k) W.tet[t][k] = n[k];
-  }
   __syncthreads();
   double nrm[D] = {0, 0, 0, 0};
+#pragma unroll 4
   for (int t = 0; t < ntet; ++t)
     for (int k = 0; k < D; ++k) nrm[k] += W.tet[t][k];
   if (lane == 0)
