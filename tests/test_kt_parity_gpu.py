@@ -148,3 +148,27 @@ def test_kernel_arrangements_agree(monkeypatch, variant, strip):
     orc, ctx = run_both(case, state, 8)
     errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI], tau=0.5 + 8 * 0.01)
     _check(errs)
+
+
+def test_full_size_benchmark_event():
+    """The benchmark's own event at its full size: 201x201x101, dx = deta =
+    0.1, tau0 = 0.5, dtau = 0.01, tabulated EoS, eta/s = 0.16, zeta/s peak
+    0.16 GeV, 200 seeded hotspots (seed 2) times the eta plateau, u at rest,
+    pi = Pi = 0.  After 4 KT steps every one of the 4.08 M cells agrees with
+    the oracle to 1e-10, and the device counts no NaN e."""
+    case = Case(201, 201, 101, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=gpuvh.EOS_TABLE,
+                zeta_tpeak=0.16 / gpuvh.HBARC)
+    e = ic.c2_event(201, 201, 101, 0.1, 0.1, 0.1, seed=2)
+    ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+    state = (e, case.pressure(e), ut, ux, uy, un, pi10, Pi)
+    print("full-size event: 4 steps on both sides", flush=True)
+    orc, ctx = run_both(case, state, 4)
+    print("full-size event: comparing", flush=True)
+    rep = []
+    errs = compare(orc, ctx, [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI], tau=0.5 + 4 * 0.01,
+                   report=rep)
+    print("full-size errors", errs)
+    _check(errs, report=rep)
+    assert ctx.diagnostics()["e_nan"] == 0
+    ctx.close()
+    orc.close()
