@@ -104,8 +104,9 @@ __device__ __forceinline__ double cube_value(const Work &W, int q, int a, int b,
 
 // Cut points of one square with values sq[a][b] at (a h1, b h2) along its
 // free axes x1 < x2, in the coordinates of those axes: up to four, in the
-// edge order below, and the outside point of each of its (nc / 2) lines.
-// Every index is fixed at compile time, so the result stays in registers.
+// edge order below, and the outside point of each of its nl lines (half the
+// number of cut points).  Every index is fixed at compile time, so the
+// result stays in registers.
 struct SquareCut {
   double cut[4][2];
   double opt[2][2];
