@@ -1,6 +1,6 @@
-"""Host-side checks of the product library that need no GPU: the C ABI
-exports, and the positional rhic-conf readers against the reference's own
-configuration files."""
+"""Host-side checks that need no GPU: the C ABI exports, the positional
+rhic-conf readers against the reference's own configuration files, and the
+bench's refusal of more than one GPU."""
 import ctypes as C
 import os
 import re
@@ -81,3 +81,17 @@ def test_positional_reader_rules(tmp_path):
 def test_missing_config_is_an_error(tmp_path):
     with pytest.raises(gpuvh.GvhError, match="no configuration file"):
         gpuvh.read_hydro_parameters(str(tmp_path))
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_bench_refuses_more_than_one_gpu(n):
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says
+    so, before touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", str(n)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and ("--gpus %d" % n) in r.stderr
+    assert r.stdout == ""
