@@ -43,22 +43,28 @@ constexpr int MAXCP = 12;   // polygons of one cube (one line each at worst)
 constexpr int MAXP = 96;    // polygons of a cell
 constexpr int MAXE = 10;    // elements of a cell (cornelius.cpp:1703, 2004)
 
+// A line: start, end, and the free-axis components of a point on the outside
+// (its two held-axis components equal the start's; the free axes are in
+// Work::oax).
 struct LineRec {
-  double s[D], e[D], o[D];  // start, end, a point on the outside
+  double s[D], e[D], o2[2];
 };
 
 struct Work {
   LineRec L[MAXL];
   double pcen[MAXP][D];
-  double pnrm[MAXCP][D];            // polygon normals (3-D cells)
+  unsigned char oax[MAXL];          // free axes x1 | x2 << 2 of each line
   unsigned char ord[MAXL];          // absolute line indices, polygon by polygon
   unsigned char pstart[MAXP], plen[MAXP], pcube[MAXP];
   unsigned char cpst[8][MAXCP], cplen[8][MAXCP];
   unsigned char hlist[MAXP], hfirst[MAXE], hlen[MAXE];
   short hntet[MAXE];
-  int cfirst[8], cnl[8], camb[8], cnp[8];
+  unsigned char cfirst[8], cnl[8], camb[8], cnp[8];
   int np, nh, nlines, overflow;
-  double tet[MAXL][D + 1];           // per-tetrahedron terms of one polyhedron
+  union {
+    double tet[MAXL][D + 1];  // per-tetrahedron terms of one polyhedron (4-D cells)
+    double pnrm[MAXCP][D];    // polygon normals (3-D cells)
+  };
   double v[16];
   double dx[D];
   double v0;
@@ -190,11 +196,12 @@ __device__ __forceinline__ SquareCut square_cut(const double sq[2][2], double h1
   return r;
 }
 
-// Line K of a square into l: free axes x1, x2 take the cut coordinates,
-// constant axes c0, c1 the values cv0, cv1.
+// Line K of a square into W.L[i]: free axes x1, x2 take the cut
+// coordinates, constant axes c0, c1 the values cv0, cv1.
 template <int K>
-__device__ __forceinline__ void store_line(LineRec &l, const SquareCut &r, int c0, int c1,
+__device__ __forceinline__ void store_line(Work &W, int i, const SquareCut &r, int c0, int c1,
                                            double cv0, double cv1, int x1, int x2) {
+  LineRec &l = W.L[i];
   l.s[x1] = r.cut[2 * K][0];
   l.s[x2] = r.cut[2 * K][1];
   l.s[c0] = cv0;
@@ -203,10 +210,16 @@ __device__ __forceinline__ void store_line(LineRec &l, const SquareCut &r, int c
   l.e[x2] = r.cut[2 * K + 1][1];
   l.e[c0] = cv0;
   l.e[c1] = cv1;
-  l.o[x1] = r.opt[K][0];
-  l.o[x2] = r.opt[K][1];
-  l.o[c0] = cv0;
-  l.o[c1] = cv1;
+  l.o2[0] = r.opt[K][0];
+  l.o2[1] = r.opt[K][1];
+  W.oax[i] = (unsigned char)(x1 | (x2 << 2));
+}
+
+// Component k of the outside point of line i.
+__device__ __forceinline__ double outside(const Work &W, int i, int k) {
+  const int a = W.oax[i];
+  const LineRec &l = W.L[i];
+  return k == (a & 3) ? l.o2[0] : (k == (a >> 2) ? l.o2[1] : l.s[k]);
 }
 
 __device__ __forceinline__ bool same_point_lt(const double *a, const double *b) {
@@ -242,8 +255,11 @@ __device__ __forceinline__ void tetra_normal(const double *a, const double *b, c
   n[3] = -1.0 / 6.0 * (a[0] * bc12 - a[1] * bc02 + a[2] * bc01);
 }
 
+__device__ __forceinline__ int pline_index(const Work &W, int g, int k) {
+  return W.ord[W.pstart[g] + k];
+}
 __device__ __forceinline__ const LineRec &pline(const Work &W, int g, int k) {
-  return W.L[W.ord[W.pstart[g] + k]];
+  return W.L[pline_index(W, g, k)];
 }
 
 // Centroid (and, for 3-D cells, normal) of polygon g.
@@ -284,7 +300,8 @@ __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
   if (!want_normal) return;
   double nsum[D] = {0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
-    const LineRec &l = pline(W, g, i);
+    const int li = pline_index(W, g, i);
+    const LineRec &l = W.L[li];
     double a[D], b[D], nrm[D], o[D];
     for (int j = 0; j < D; ++j) {
       a[j] = l.s[j] - cen[j];
@@ -294,7 +311,7 @@ __device__ void polygon_geometry(Work &W, int g, bool want_normal) {
     const double n2 = -0.5 * (pick(a, x1) * pick(b, x3) - pick(a, x3) * pick(b, x1));
     const double n3 = 0.5 * (pick(a, x1) * pick(b, x2) - pick(a, x2) * pick(b, x1));
     for (int j = 0; j < D; ++j) nrm[j] = j == x1 ? n1 : (j == x2 ? n2 : (j == x3 ? n3 : 0.0));
-    for (int j = 0; j < D; ++j) o[j] = l.o[j] - cen[j];
+    for (int j = 0; j < D; ++j) o[j] = outside(W, li, j) - cen[j];
     orient(nrm, o);
     for (int j = 0; j < D; ++j) nsum[j] += nrm[j];
   }
@@ -326,13 +343,14 @@ __device__ __forceinline__ void tet_line(const Work &W, int first, int np, int t
 __device__ void polyhedron_geometry(Work &W, int h, int lane, double *cen_out, double *nrm_out) {
   const int first = W.hfirst[h], np = W.hlen[h], ntet = W.hntet[h];
   static_assert(MAXL <= 128, "two tetrahedra per lane");
-  int g[2] = {0, 0}, j[2] = {0, 0};
+  int g[2] = {0, 0}, j[2] = {0, 0}, li[2] = {0, 0};
   const LineRec *ln[2] = {nullptr, nullptr};
 #pragma unroll
   for (int r = 0; r < 2; ++r)
     if (lane + 64 * r < ntet) {
       tet_line(W, first, np, lane + 64 * r, g[r], j[r]);
-      ln[r] = &pline(W, g[r], j[r]);
+      li[r] = pline_index(W, g[r], j[r]);
+      ln[r] = &W.L[li[r]];
     }
   // mean of the line end points
 #pragma unroll
@@ -387,7 +405,7 @@ __device__ void polyhedron_geometry(Work &W, int h, int lane, double *cen_out, d
         a[k] = l.s[k] - cen[k];
         b[k] = l.e[k] - cen[k];
         c[k] = pc[k] - cen[k];
-        o[k] = l.o[k] - cen[k];
+        o[k] = outside(W, li[r], k) - cen[k];
       }
       tetra_normal(a, b, c, n);
       orient(n, o);
@@ -462,8 +480,8 @@ __device__ int cell_elements(Work &W, double (*cen)[D], double (*nrm)[D]) {
   const unsigned long long b1 = __ballot(cnt >= 1), b2 = __ballot(cnt == 2);
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int pos = __popcll(b1 & lt) + __popcll(b2 & lt);
-  if (cnt >= 1) store_line<0>(W.L[pos], sc, c0, c1, cv0, cv1, x1, x2);
-  if (cnt == 2) store_line<1>(W.L[pos + 1], sc, c0, c1, cv0, cv1, x1, x2);
+  if (cnt >= 1) store_line<0>(W, pos, sc, c0, c1, cv0, cv1, x1, x2);
+  if (cnt == 2) store_line<1>(W, pos + 1, sc, c0, c1, cv0, cv1, x1, x2);
   if (lane < ncube) {
     const unsigned long long below = (1ull << (6 * lane)) - 1ull;
     const unsigned long long mine = 0x3full << (6 * lane);
