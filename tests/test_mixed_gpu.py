@@ -1,12 +1,12 @@
-"""Mixed-precision form (FP32 stencil, limiter, flux, sources and pi/Pi
-storage; FP64 T^{tau mu} accumulation, root-finds and EoS) against the FP64
-form, which matches the oracle to 1e-10.
+"""Mixed-precision form (FP32 stencil, limiter, flux, half-site root-finds,
+sources and pi/Pi storage; FP64 T^{tau mu} accumulation and the cell's own
+root-find) against the FP64 form, which matches the oracle to 1e-10.
 
 Metric: per-cell relative error with a floor of MIXED_FLOOR (1%) of the
 field's scale -- components that cross zero (u^x, u^y, u^eta, pi) are
 resolved to FP32 precision in absolute terms, so below 1% of their maximum
 the bar becomes absolute (1e-6 of the maximum).  Bound: 1e-4 for e and u^mu
-1e-3 for pi^{mu nu}.  The Lorentz-norm error
+(the tolerance stated for the mixed form), 1e-3 for pi^{mu nu}.  The Lorentz-norm error
 max |du^mu| / u^tau is reported beside it."""
 import numpy as np
 import pytest
