@@ -11,9 +11,10 @@ namespace gvh {
 // a time, 2 separate source kernel + flux kernel (states rebuilt), 3 as 0
 // with the four root-finds of a direction in lockstep, 4 as 3 with the
 // states rebuilt for the flux after the root-finds.  The mixed form
-// (FP32 stencil, limiter, flux and sources, FP32 pi/Pi storage, FP64
-// T^{tau mu}, accumulator, root-finds and EoS) has arrangements 0, 3 and 4;
-// others run as 0.
+// (FP32 stencil, limiter, flux and sources, FP32 pi/Pi storage, FP32
+// half-site root-finds for the tabulated and conformal EoS; FP64
+// T^{tau mu}, accumulator, the cell's own root-find and its EoS) has
+// arrangements 0, 3 and 4; others run as 0.
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool MIXED>
 static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
   const long long ncell = (long long)A.nx * A.strip * A.nstrips * A.nz;

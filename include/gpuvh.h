@@ -81,7 +81,9 @@ int gvh_read_ic_parameters(const char *config_dir, gvh_ic_parameters *out);
 
 #define GVH_FP64 0  /* plain FP64 everywhere                                   */
 #define GVH_MIXED 1 /* FP32 stencil/flux/source and FP32 pi, Pi storage;
-                       FP64 T^{tau mu} accumulation, root-find and EoS       */
+                       FP32 half-site root-finds (tabulated and conformal
+                       EoS); FP64 T^{tau mu} accumulation, the cell's own
+                       root-find and its EoS                                  */
 
 typedef struct gvh_options {
   int eos;             /* GVH_EOS_*                                         */
