@@ -391,6 +391,41 @@ def test_source_terms_covariant(eos, bulk):
         assert np.allclose(S, S_ref, rtol=1e-9, atol=1e-11 * scale), (S - S_ref)
 
 
+@pytest.mark.parametrize("eos,bulk", [(O.EOS_WB, True), (O.EOS_TABLE, True),
+                                      (O.EOS_CONFORMAL, False)])
+def test_source_terms_covariant_general_inputs(eos, bulk):
+    """kto_source_cell == the covariant restatement off the physical set
+    the lattice only approximates: pi any symmetric tensor (neither
+    traceless nor transverse, as after regulation or rounding), spatial
+    gradients of u^tau unconstrained by the normalisation, and a previous
+    velocity that is not normalised.  Every term of the reference's
+    second-order expression enters the comparison with an O(1) weight."""
+    etabar, dt = 0.16, 0.01
+    P = params(eos, bulk=bulk, etabar=etabar, zeta_tpeak=0.16 / HBARC, dt=dt)
+    rng = np.random.default_rng(15)
+    for _ in range(200):
+        t = rng.uniform(0.4, 4.0)
+        u = unit_u(t, rng.uniform(-0.7, 0.7), rng.uniform(-0.7, 0.7), rng.uniform(-0.4, 0.4) / t)
+        up = u - 0.01 * rng.standard_normal(4) * np.array([1.0, 1.0, 1.0, 1.0 / t])
+        du = rng.standard_normal((3, 4)) * np.array([1.0, 1.0, 1.0, 1.0 / t])
+        e = 10 ** rng.uniform(0, 2)
+        p = O.eos(P, e)[0]
+        Pi = -0.2 * p * rng.uniform() if bulk else 0.0
+        dp = rng.standard_normal(3) * e
+        A = rng.standard_normal((4, 4))
+        pi = 0.5 * (A + A.T) * 0.2 * e
+        pi[:, 3] /= t
+        pi[3, :] /= t
+        S_ref, Tmn = covariant_source(P, t, e, p, Pi, u, up, du, dp, pi, dt, etabar, bulk)
+        Q = np.zeros(15)
+        Q[0:4] = Tmn[0]
+        Q[4:14] = pi_components(pi)
+        Q[14] = Pi
+        S = O.source_cell(P, t, Q, e, p, u, up, du, dp)
+        scale = np.abs(S_ref).max()
+        assert np.allclose(S, S_ref, rtol=1e-9, atol=1e-11 * scale), (S - S_ref)
+
+
 # ----------------------------------------------- Bjorken flow, RK2 order
 def bjorken_ode(P, etabar, bulk):
     """Boost-invariant flow (u = (1,0,0,0)) of the covariant equations:

@@ -9,10 +9,12 @@ mathematics fixes:
   omega^{mu nu}, I2, I3, I4, J^{mu nu} from nabla_mu u^nu with Christoffel
   symbols) for random states, flows with u^eta included.  This kernel keeps
   an older vorticity convention (HydrodynamicValidity.cu:116-121): it equals
-  minus the covariant omega wherever u.d u^tau - tau (u^eta)^2 = 0 and
-  D u^eta = 0, and the states are drawn on that set; off it the older
-  expressions carry extra terms in (u^eta d u^tau) and D u^eta that no
-  independent form fixes (parity unpinned for those terms);
+  minus the covariant omega plus terms along the acceleration,
+  Delta^tau_eta = tau^2 u^eta (Du^tau - tau (u^eta)^2)/2 + (3/2) u^tau tau^2 Du^eta,
+  Delta^x_eta = u^x tau^2 Du^eta, Delta^y_eta = u^y tau^2 Du^eta (the
+  difference of the reference's expressions from the covariant ones, written
+  with the covariant acceleration Du^mu = u^nu nabla_nu u^mu); states on the
+  set where these vanish and general states are both drawn;
 - the lattice glue: central differences of linear velocity profiles are the
   slopes, and pi, Pi come from q^n of the last step.
 
@@ -75,11 +77,11 @@ def test_validity_bjorken_closed_form(eos, bulk):
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
 
 
-def covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt):
+def covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt, older_vorticity=False):
     """The validity numbers from the covariant tensors.  nab[mu, nu] =
     nabla_mu u^nu with Gamma^tau_{eta eta} = tau, Gamma^eta_{tau eta} = 1/tau;
     sigma and omega from the transverse gradient; the kernel's vorticity
-    is -omega on the states drawn here."""
+    is -omega, plus its terms along the acceleration when older_vorticity."""
     g, gi = metric(t)
     Gam = np.zeros((4, 4, 4))
     Gam[0, 3, 3] = t
@@ -95,6 +97,18 @@ def covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt):
     sigma = 0.5 * (nperp + nperp.T) - Delta * theta / 3.0
     omega_kernel = -0.5 * (nperp - nperp.T)
     om_mixed = omega_kernel @ g
+    if older_vorticity:
+        # the kernel's extra terms along the acceleration, as omega^mu_nu
+        ut, ux, uy, un = u
+        a_eta = t * t * Du[3]
+        dlt = np.zeros((4, 4))
+        dlt[0, 3] = t * t * un * (Du[0] - t * un * un) / 2 + 1.5 * ut * a_eta
+        dlt[1, 3] = ux * a_eta
+        dlt[2, 3] = uy * a_eta
+        dlt[3, 0] = dlt[0, 3] / (t * t)
+        dlt[3, 1] = -dlt[1, 3] / (t * t)
+        dlt[3, 2] = -dlt[2, 3] / (t * t)
+        om_mixed = om_mixed + dlt
     sg_mixed = sigma @ g
     I2 = theta * pi
     I3 = pi @ om_mixed.T + (pi @ om_mixed.T).T
@@ -140,6 +154,37 @@ def test_validity_covariant(eos, bulk):
         want = covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt)
         got = O.validity_cell(P, t, e, p, u, up, du, pi_components(pi), Pi)
         np.testing.assert_allclose(got, want, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("eos,bulk", [(O.EOS_WB, True), (O.EOS_TABLE, True),
+                                      (O.EOS_CONFORMAL, False)])
+def test_validity_older_vorticity_general_flow(eos, bulk):
+    """kto_validity_cell == the covariant restatement with the older
+    vorticity's acceleration terms, for random normalised u (u^eta included),
+    random spatial gradients and random time derivatives (so Du^tau and
+    Du^eta are arbitrary); transverse traceless pi.  The restatement without
+    those terms must miss (the states resolve them)."""
+    dt = 0.01
+    P = params(eos, bulk=bulk, etabar=0.16, zeta_tpeak=0.16 / HBARC, dt=dt)
+    rng = np.random.default_rng(13)
+    resolved = 0
+    for _ in range(100):
+        t = rng.uniform(0.4, 4.0)
+        u = unit_u(t, rng.uniform(-0.7, 0.7), rng.uniform(-0.7, 0.7), rng.uniform(-0.4, 0.4) / t)
+        du = rng.standard_normal((3, 4)) * np.array([1.0, 1.0, 1.0, 1.0 / t])
+        dtu = rng.standard_normal(4) * np.array([1.0, 1.0, 1.0, 1.0 / t])
+        up = u - dt * dtu
+        e = 10 ** rng.uniform(0, 2)
+        p = O.eos(P, e)[0]
+        Pi = -0.2 * p * rng.uniform() if bulk else 0.0
+        pi = transverse_traceless(t, u, rng, 0.2 * e)
+        want = covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt, older_vorticity=True)
+        got = O.validity_cell(P, t, e, p, u, up, du, pi_components(pi), Pi)
+        np.testing.assert_allclose(got, want, rtol=1e-8, atol=0)
+        plain = covariant_validity(P, t, e, p, u, up, du, pi, Pi, dt)
+        if abs(plain[O.V_RE2_PI] / got[O.V_RE2_PI] - 1) > 1e-6:
+            resolved += 1
+    assert resolved > 90
 
 
 def test_validity_grid_differences_and_buffers():
