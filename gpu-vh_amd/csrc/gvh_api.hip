@@ -159,6 +159,17 @@ static int check_dims(const gvh_lattice_parameters *L) {
   return GVH_OK;
 }
 
+int check_event_params(const gvh_lattice_parameters *L, const gvh_hydro_parameters *H) {
+  int rc = check_dims(L);
+  if (rc) return rc;
+  if (!H) return gvh_fail(GVH_EINVAL, "hydro parameters are NULL");
+  if (!(H->shearViscosityToEntropyDensity > 0))
+    return gvh_fail(GVH_EINVAL, "eta/s must be positive");
+  if (!(H->initialProperTimePoint > 0))
+    return gvh_fail(GVH_EINVAL, "the initial proper time must be positive");
+  return GVH_OK;
+}
+
 int eos_from_options(const gvh_options *o, HostEos &H) {
   H.kind = o ? o->eos : EOS_WB;
   if (H.kind == EOS_TABLE) {
@@ -226,11 +237,8 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
                           const gvh_options *opts, gvh_context **out) {
   if (!out) return gvh_fail(GVH_EINVAL, "out is NULL");
   *out = nullptr;
-  int rc = check_dims(lattice);
+  int rc = check_event_params(lattice, hydro);
   if (rc) return rc;
-  if (!hydro) return gvh_fail(GVH_EINVAL, "hydro parameters are NULL");
-  if (!(hydro->shearViscosityToEntropyDensity > 0))
-    return gvh_fail(GVH_EINVAL, "eta/s must be positive");
   if (opts && opts->precision != GVH_FP64 && opts->precision != GVH_MIXED)
     return gvh_fail(GVH_EINVAL, "unknown precision %d", opts->precision);
 
@@ -551,6 +559,7 @@ StageArgs make_stage_args(gvh_context *c, int stage, double t) {
   A.strip = c->strip;
   A.nstrips = (c->ny + c->strip - 1) / c->strip;
   A.variant = c->variant;
+  A.max_iters = c->max_iters;
   return A;
 }
 }  // namespace gvh

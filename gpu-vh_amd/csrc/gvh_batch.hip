@@ -59,6 +59,7 @@ struct Pending {
   gvh_hydro_parameters H;
   std::vector<double> e;
   int surface, keep_state, fo_flags;
+  int newton_budget;  // 0: MAX_ITERS
   double t_submit;
   size_t footprint;
 };
@@ -292,7 +293,14 @@ int tune_context(gvh_context *c, int steps, int *variant, int *strip, float *bes
   const size_t bytes = (3 * c->qstride + 16 * (size_t)c->G) * sizeof(double);
   void *save = nullptr;
   HIP_TRY(hipMalloc(&save, bytes));
-  hipError_t e = hipMemcpyAsync(save, c->arena, bytes, hipMemcpyDeviceToDevice, c->stream);
+  // the tuning steps' failure counters are not the event's: kept aside and
+  // put back with the state
+  unsigned long long diag_save[DIAG_N];
+  hipError_t e = hipMemcpyAsync(diag_save, c->diag, sizeof diag_save, hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(save, c->arena, bytes, hipMemcpyDeviceToDevice, c->stream);
   const int variants[2] = {0, 3};
   const int strips[5] = {4, 8, 16, 32, 64};
   const int v0 = c->variant, s0 = c->strip, iq = c->iq, iu = c->iu, iup = c->iup;
@@ -324,6 +332,8 @@ int tune_context(gvh_context *c, int steps, int *variant, int *strip, float *bes
       c->iup = iup;
       c->steps_done = sd;
     }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->diag, diag_save, sizeof diag_save, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(save);
   c->variant = bv;
@@ -334,6 +344,23 @@ int tune_context(gvh_context *c, int steps, int *variant, int *strip, float *bes
   if (strip) *strip = bs;
   if (best_ms) *best_ms = best / steps;
   return GVH_OK;
+}
+
+// An event whose admission failed: finished at once with the error code in
+// its result, so that the events queued behind it and the running ones go on.
+void fail_pending(gvh_batch *b, const Pending &P, int rc) {
+  Finished F;
+  F.r.id = P.id;
+  F.r.steps = 0;
+  F.r.frozen = 0;
+  F.r.t_final = P.H.initialProperTimePoint;
+  F.r.surface_rows = 0;
+  F.r.t_submit = P.t_submit;
+  F.r.t_admit = F.r.t_done = seconds(b->origin, Clock::now());
+  F.r.status = rc;
+  F.r.newton_not_converged = 0;
+  b->done[P.id] = std::move(F);
+  b->unpolled.push_back(P.id);
 }
 
 int admit(gvh_batch *b) {
@@ -351,7 +378,12 @@ int admit(gvh_batch *b) {
     o.stream = b->stream;
     gvh_context *c = nullptr;
     int rc = gvh_create(&P.L, &P.H, &o, &c);
-    if (rc) return rc;
+    if (rc) {
+      fail_pending(b, P, rc);
+      b->queue.pop_front();
+      continue;
+    }
+    if (P.newton_budget > 0 && P.newton_budget < MAX_ITERS) c->max_iters = P.newton_budget;
     const size_t n = P.e.size();
     std::vector<double> p(n), ut(n, 1.0), zero(n, 0.0);
     for (size_t i = 0; i < n; ++i) p[i] = host_eos_p(c->eos, P.e[i]);
@@ -377,7 +409,9 @@ int admit(gvh_batch *b) {
     if (!rc && P.surface) rc = gvh_freezeout_init(c, b->fofreq, e_fo, P.fo_flags);
     if (rc) {
       gvh_destroy(c);
-      return rc;
+      fail_pending(b, P, rc);
+      b->queue.pop_front();
+      continue;
     }
     BatchSlot S{};
     S.A = make_stage_args(c, 1, t0);
@@ -437,6 +471,11 @@ int retire(gvh_batch *b, int s) {
   F.r.t_submit = L.t_submit;
   F.r.t_admit = L.t_admit;
   F.r.t_done = seconds(b->origin, Clock::now());
+  F.r.status = S.error ? GVH_ENOCONV : GVH_OK;
+  gvh_diagnostics dg{};
+  int rc = gvh_get_diagnostics(c, &dg);
+  if (rc) return rc;
+  F.r.newton_not_converged = dg.newton_not_converged;
   F.rows = std::move(L.rows);
   if (L.keep) {
     F.ctx = c;
@@ -477,6 +516,13 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
     return gvh_fail(GVH_EINVAL, "max_events must be in [1, 4096]");
   if (B.fofreq < 3 || B.fofreq > GVH_FO_MAX_FREQ)
     return gvh_fail(GVH_EINVAL, "fofreq must be in [3, %d]", GVH_FO_MAX_FREQ);
+  {  // the options every event's context will be created with
+    HostEos H;
+    const int rc = eos_from_options(opts, H);
+    if (rc) return rc;
+    if (opts->precision != GVH_FP64 && opts->precision != GVH_MIXED)
+      return gvh_fail(GVH_EINVAL, "unknown precision %d", opts->precision);
+  }
   gvh_batch *b = new gvh_batch();
   b->opts = *opts;
   // keep the tabulated EoS alive for the contexts created later
@@ -540,9 +586,10 @@ extern "C" int gvh_batch_destroy(gvh_batch *b) {
 extern "C" int gvh_batch_submit(gvh_batch *b, const gvh_event *ev, long long *id) {
   if (!b || !ev || !ev->e) return gvh_fail(GVH_EINVAL, "gvh_batch_submit: NULL argument");
   const gvh_lattice_parameters &L = ev->lattice;
-  if (L.numLatticePointsX < 1 || L.numLatticePointsY < 1 || L.numLatticePointsRapidity < 1)
-    return gvh_fail(GVH_EINVAL, "lattice dimensions must be positive");
+  int rc = check_event_params(&L, &ev->hydro);
+  if (rc) return rc;
   if (L.numProperTimePoints < 0) return gvh_fail(GVH_EINVAL, "numProperTimePoints must be >= 0");
+  if (ev->newton_budget < 0) return gvh_fail(GVH_EINVAL, "newton_budget must be >= 0");
   if (ev->surface && !(L.numLatticePointsX > 1 && L.numLatticePointsY > 1))
     return gvh_fail(GVH_EINVAL, "the surface needs a 2+1D or 3+1D lattice");
   Pending P;
@@ -557,6 +604,7 @@ extern "C" int gvh_batch_submit(gvh_batch *b, const gvh_event *ev, long long *id
   P.surface = ev->surface;
   P.keep_state = ev->keep_state;
   P.fo_flags = ev->fo_flags;
+  P.newton_budget = ev->newton_budget;
   P.t_submit = seconds(b->origin, Clock::now());
   P.footprint = event_footprint(P.L, b->opts.precision, P.surface, b->fofreq);
   if (P.footprint > b->budget)

@@ -83,12 +83,19 @@ __global__ __launch_bounds__(BLOCK) void batch_record_kernel(BatchSlot *__restri
 
 // Termination before the step (HydroPlugin.cpp:239, 396-413): an event ends
 // after nt + 1 steps, or once FOFREQ + 1 recorded states had no cell above
-// e_fo.
+// e_fo.  An event with a Newton iteration budget ends, with its error flag
+// set, after a step in which a root-find reached the budget (the reference
+// has no budget: it iterates up to MAX_ITERS, EnergyMomentumTensor.cu:22-44).
 __global__ void batch_check_kernel(BatchSlot *__restrict__ slots, int nslots) {
   const int sl = blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslots) return;
   BatchSlot &S = slots[sl];
   if (S.status != SLOT_RUN) return;
+  if (S.A.max_iters < MAX_ITERS && S.A.diag[DIAG_NOT_CONVERGED] != 0) {
+    S.status = SLOT_DONE;
+    S.error = 1;
+    return;
+  }
   if (S.steps >= S.max_steps) {
     S.status = SLOT_DONE;
     return;

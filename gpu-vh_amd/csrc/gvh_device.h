@@ -72,6 +72,8 @@ struct StageArgs {
   EosTable tab;
   int strip, nstrips;   // y rows per traversal strip, number of strips
   int variant;          // kernel arrangement of a stage (gvh_stage.hip)
+  int max_iters;        // Newton iterations per root-find: MAX_ITERS (the reference's),
+                        // or a smaller budget of a batched event
 };
 
 // Buffers of one event and their current roles: the reference's pointer
@@ -133,6 +135,7 @@ struct BatchSlot {
   int fofreq;
   int fo_flags;       // GVH_FO_CURRENT_FIELDS selects the current u, pi for the ring
   int frozen_stop;    // 1 if it ended by the all-frozen rule
+  int error;          // 1 if it was retired for a root-find over its iteration budget
   int bulk;           // Pi is evolved
   int tile_first, ntiles;  // its tiles (blocks of cells) in the tile map
   double e_fo;
@@ -418,10 +421,12 @@ __device__ __forceinline__ bool newton_step(const EosTable &T, F &e0, F M0, F M,
   return newton_update<DAMPED, F>(e0, p, cs2, M0, M, Pi);
 }
 
+// max_iters <= MAX_ITERS bounds the iterations (the reference: MAX_ITERS);
+// the damping still starts at MAX_ITERS/2.
 template <int EOS>
 __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, double M0,
                                           double M, double Pi,
-                                          unsigned long long *diag) {
+                                          unsigned long long *diag, int max_iters) {
   if constexpr (EOS == EOS_CONFORMAL) {
     return fabs(sqrt(fabs(4.0 * M0 * M0 - 3.0 * M)) - M0);
   } else {
@@ -430,11 +435,12 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
     // first NaN returns the same value.
     double e = ePrev;
     bool nan = false;
-    for (int it = 0; it < MAX_ITERS / 2 && !nan; ++it) {
+    const int lim1 = max_iters < MAX_ITERS / 2 ? max_iters : MAX_ITERS / 2;
+    for (int it = 0; it < lim1 && !nan; ++it) {
       if (newton_step<EOS, false>(T, e, M0, M, Pi)) return e;
       nan = isnan(e);
     }
-    for (int it = MAX_ITERS / 2; it < MAX_ITERS && !nan; ++it) {
+    for (int it = MAX_ITERS / 2; it < max_iters && !nan; ++it) {
       if (newton_step<EOS, true>(T, e, M0, M, Pi)) return e;
       nan = isnan(e);
     }
@@ -452,7 +458,7 @@ __device__ __forceinline__ double solve_e(const EosTable &T, double ePrev, doubl
 template <int EOS, int N, typename F = double>
 __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, const F *M0,
                                                  const F *M, const F *Pi, F *e,
-                                                 unsigned long long *diag) {
+                                                 unsigned long long *diag, int max_iters) {
   if constexpr (EOS == EOS_CONFORMAL) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -464,7 +470,8 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, con
 #pragma unroll
     for (int k = 0; k < N; ++k) e[k] = ePrev;
     unsigned failed = 0;  // chains that reached a NaN iterate (see solve_e)
-    for (int it = 0; it < MAX_ITERS / 2 && act; ++it) {
+    const int lim1 = max_iters < MAX_ITERS / 2 ? max_iters : MAX_ITERS / 2;
+    for (int it = 0; it < lim1 && act; ++it) {
       // the N table gathers first, all in flight together
       F pk[N], ck[N];
 #pragma unroll
@@ -482,7 +489,7 @@ __device__ __forceinline__ void solve_e_lockstep(const EosTable &T, F ePrev, con
         }
       }
     }
-    for (int it = MAX_ITERS / 2; it < MAX_ITERS && act; ++it) {
+    for (int it = MAX_ITERS / 2; it < max_iters && act; ++it) {
       // the N table gathers first, all in flight together
       F pk[N], ck[N];
 #pragma unroll
@@ -516,11 +523,11 @@ __device__ __forceinline__ Prim inferred(const EosTable &T, double t, double ttt
                                          double ttx, double tty, double ttn,
                                          double pitt, double pitx, double pity,
                                          double pitn, double Pi, double ePrev,
-                                         unsigned long long *diag) {
+                                         unsigned long long *diag, int max_iters) {
   double M0 = ttt - pitt, M1 = ttx - pitx, M2 = tty - pity, M3 = ttn - pitn;
   double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
   Prim r;
-  r.e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag);
+  r.e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag, max_iters);
   if (isnan(r.e)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
   r.p = eos_p<EOS>(T, r.e);
   if (r.e < 1.e-7) {
@@ -549,10 +556,10 @@ __device__ __forceinline__ double flux_velocity(const EosTable &T, double t, dou
                                                 double ttx, double tty, double ttn, double pitt,
                                                 double pitx, double pity, double pitn,
                                                 double Pi, double ePrev,
-                                                unsigned long long *diag) {
+                                                unsigned long long *diag, int max_iters) {
   const double M0 = ttt - pitt, M1 = ttx - pitx, M2 = tty - pity, M3 = ttn - pitn;
   const double M = M1 * M1 + M2 * M2 + t * t * M3 * M3;
-  double e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag);
+  double e = (ePrev <= 0.1) ? M0 - M / M0 : solve_e<EOS>(T, ePrev, M0, M, Pi, diag, max_iters);
   if (isnan(e)) atomicAdd(&diag[DIAG_E_NAN], 1ull);
   double p = eos_p<EOS>(T, e);
   if (e < 1.e-7) {
@@ -579,7 +586,8 @@ using flux_root_t =
 template <int EOS, int DIR, int N, typename R>
 __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double t,
                                                        const R (*W)[9], double ePrev,
-                                                       unsigned long long *diag, double *v) {
+                                                       unsigned long long *diag, int max_iters,
+                                                       double *v) {
   using F = flux_root_t<EOS, R>;
   F M0[N], M[N], Md[N], Pi[N], e[N];
   const F tf = F(t);
@@ -596,7 +604,7 @@ __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double
 #pragma unroll
     for (int k = 0; k < N; ++k) e[k] = M0[k] - M[k] / M0[k];
   } else {
-    solve_e_lockstep<EOS, N, F>(T, F(ePrev), M0, M, Pi, e, diag);
+    solve_e_lockstep<EOS, N, F>(T, F(ePrev), M0, M, Pi, e, diag, max_iters);
   }
   // the N pressure gathers first, so that they are in flight together (an
   // atomic between them would order them)

@@ -19,6 +19,8 @@ struct HostEos {
 
 double host_e_of_T(const HostEos &H, double T);
 int eos_from_options(const gvh_options *o, HostEos &H);
+// Argument checks shared by gvh_create and gvh_batch_submit.
+int check_event_params(const gvh_lattice_parameters *L, const gvh_hydro_parameters *H);
 int upload_eos_table(const HostEos &H, void **dev_pc, void **dev_tt, EosTable &T, hipStream_t st);
 double host_eos_p(const HostEos &H, double e);
 
@@ -90,6 +92,7 @@ struct gvh_context {
   int ie = 0;   // (e, p) buffer holding the current state
   int variant = 3;  // stage kernel arrangement (gvh_stage.hip launch_z)
   int strip = 16;   // y rows per traversal strip
+  int max_iters = gvh::MAX_ITERS;  // Newton iteration budget per root-find
   bool initialized = false;
   long long steps_done = 0;     // KT steps since the initial state was set
   gvh::FoState *fo = nullptr;   // freeze-out history and surface buffers

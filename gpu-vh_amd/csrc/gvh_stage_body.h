@@ -339,14 +339,14 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     if constexpr (EARLY_RAW2) load_raw2();
     double v4[4];
     if constexpr (LOCK) {
-      flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, v4);
+      flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, A.max_iters, v4);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         v4[k] = flux_velocity<EOS, DIR>(A.tab, t, (double)Wst[k][0], (double)Wst[k][1],
                                         (double)Wst[k][2], (double)Wst[k][3], (double)Wst[k][4],
                                         (double)Wst[k][5], (double)Wst[k][6], (double)Wst[k][7],
-                                        (double)Wst[k][8], ePrev, A.diag);
+                                        (double)Wst[k][8], ePrev, A.diag, A.max_iters);
     }
     vRf = v4[0];
     vLf = v4[1];
@@ -373,7 +373,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
       const double v = flux_velocity<EOS, DIR>(A.tab, t, (double)w[0], (double)w[1], (double)w[2],
                                                (double)w[3], (double)w[4], (double)w[5],
                                                (double)w[6], (double)w[7], (double)w[8], ePrev,
-                                               A.diag);
+                                               A.diag, A.max_iters);
       vRf = state == 0 ? v : vRf;
       vLf = state == 1 ? v : vLf;
       vRb = state == 2 ? v : vRb;
@@ -628,7 +628,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   // inferred variables with this cell's e as the guess
   const Prim pr = inferred<EOS>(A.tab, A.tinf, acc[0], acc[BLOCK], acc[2 * BLOCK], acc[3 * BLOCK],
                                 acc[4 * BLOCK], acc[5 * BLOCK], acc[6 * BLOCK], acc[7 * BLOCK],
-                                BULK ? acc[14 * BLOCK] : 0.0, e_s, A.diag);
+                                BULK ? acc[14 * BLOCK] : 0.0, e_s, A.diag, A.max_iters);
 
   // regulateDissipativeCurrents (RegulateDissipativeCurrents.cu:16-134): only
   // cells at or below e_reg are rescaled; the bulk factor is 1 unless p is 0.

@@ -22,6 +22,8 @@ HBARC = 0.197326938  # GeV fm (rhic/src/HydroPlugin.cpp:132)
 
 EOS_CONFORMAL, EOS_WB, EOS_TABLE = 0, 1, 2
 FP64, MIXED = 0, 1
+# status codes (include/gpuvh.h GVH_*)
+OK, EINVAL, EHIP, EIO, ENOMEM, ENOCONV = 0, -1, -2, -3, -4, -5
 
 # field ids, include/gpuvh.h
 (TTT, TTX, TTY, TTN, PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, PIYN, PINN, PI,
@@ -113,13 +115,14 @@ class BatchOptions(C.Structure):
 class Event(C.Structure):
     _fields_ = [("lattice", LatticeParameters), ("hydro", HydroParameters),
                 ("e", C.POINTER(C.c_double)), ("surface", C.c_int), ("fo_flags", C.c_int),
-                ("keep_state", C.c_int)]
+                ("keep_state", C.c_int), ("newton_budget", C.c_int)]
 
 
 class EventResult(C.Structure):
     _fields_ = [("id", C.c_longlong), ("steps", C.c_int), ("frozen", C.c_int),
                 ("t_final", C.c_double), ("surface_rows", C.c_longlong),
-                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("status", C.c_int), ("newton_not_converged", C.c_ulonglong)]
 
 
 class Diagnostics(C.Structure):
@@ -335,10 +338,19 @@ class Context:
         return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
 
     def set_initial_state(self, t0, e, p, ut, ux, uy, un, pi10=None, Pi=None):
+        n = self.nx * self.ny * self.nz
         arrs = [self._flat(a) for a in (e, p, ut, ux, uy, un)]
         pi = None if pi10 is None else np.ascontiguousarray(
             np.stack([self._flat(a) for a in pi10]))
         Pi_a = None if Pi is None else self._flat(Pi)
+        # the library reads exactly n values per field (10 n for pi)
+        for name, a in zip(("e", "p", "ut", "ux", "uy", "un"), arrs):
+            if a.size != n:
+                raise ValueError("%s has %d values, the lattice %d" % (name, a.size, n))
+        if pi is not None and pi.shape != (10, n):
+            raise ValueError("pi10 must hold 10 fields of %d values, got %s" % (n, pi.shape))
+        if Pi_a is not None and Pi_a.size != n:
+            raise ValueError("Pi has %d values, the lattice %d" % (Pi_a.size, n))
         _check(lib().gvh_set_initial_state(self.h, t0, *[_ptr(a) for a in arrs],
                                            _ptr(pi), _ptr(Pi_a)))
 
@@ -499,12 +511,21 @@ class Batch:
         except Exception:
             pass
 
-    def submit(self, lattice_params, hydro_params, e, surface=False, fo_flags=0, keep_state=False):
+    def submit(self, lattice_params, hydro_params, e, surface=False, fo_flags=0, keep_state=False,
+               newton_budget=0):
+        """Queue an event; newton_budget > 0 retires it with status
+        ENOCONV after a step in which a root-find reached that many
+        iterations (0: the reference's limit)."""
         ev = Event()
         ev.lattice, ev.hydro = lattice_params, hydro_params
         a = Context._flat(e)
+        n = (lattice_params.numLatticePointsX * lattice_params.numLatticePointsY
+             * lattice_params.numLatticePointsRapidity)
+        if a.size != n:
+            raise ValueError("e has %d values, the lattice %d" % (a.size, n))
         ev.e = _ptr(a)
         ev.surface, ev.fo_flags, ev.keep_state = int(surface), fo_flags, int(keep_state)
+        ev.newton_budget = int(newton_budget)
         i = C.c_longlong()
         _check(lib().gvh_batch_submit(self.h, C.byref(ev), C.byref(i)))
         self.shapes[i.value] = (lattice_params.numLatticePointsX, lattice_params.numLatticePointsY,

@@ -21,6 +21,8 @@ extern "C" {
 #define GVH_EHIP (-2)     /* HIP runtime failure                       */
 #define GVH_EIO (-3)      /* file missing or malformed                 */
 #define GVH_ENOMEM (-4)   /* device allocation failed                  */
+#define GVH_ENOCONV (-5)  /* a batched event's root-find reached its
+                             Newton iteration budget (event retired)  */
 
 /* Parameter blocks: same fields in the same order as the reference's POD
  * structs, so a caller holding one can pass it field for field. */
@@ -336,6 +338,14 @@ typedef struct gvh_event {
   int surface;                    /* find the freeze-out surface              */
   int fo_flags;                   /* GVH_FO_* for the surface rows            */
   int keep_state;                 /* keep the final state for gvh_batch_field */
+  int newton_budget;              /* 0: the reference's 10^6 Newton iterations per
+                                     root-find (EnergyMomentumTensor.cu:15,22);
+                                     n > 0: at most n, and the event is retired
+                                     with status GVH_ENOCONV after the first step
+                                     in which a root-find reached n without
+                                     converging, so that an unstable event does
+                                     not hold its co-batched events to 10^6-
+                                     iteration chains                         */
 } gvh_event;
 
 typedef struct gvh_event_result {
@@ -345,12 +355,20 @@ typedef struct gvh_event_result {
   double t_final;           /* proper time at the end                      */
   long long surface_rows;   /* rows found (gvh_batch_rows)                 */
   double t_submit, t_admit, t_done; /* seconds on the batch's clock        */
+  int status;               /* GVH_OK; GVH_ENOCONV (iteration budget
+                               reached); or the error code of a failed
+                               admission (the event never ran)         */
+  unsigned long long newton_not_converged; /* root-finds that reached the
+                                              iteration limit            */
 } gvh_event_result;
 
 void gvh_batch_options_default(gvh_batch_options *bo);
 int gvh_batch_create(const gvh_options *opts, const gvh_batch_options *bo, gvh_batch **out);
 int gvh_batch_destroy(gvh_batch *b);
-/* Queue an event (its arrays are copied). */
+/* Queue an event (its arrays are copied).  The lattice and hydro
+ * parameters are checked as gvh_create checks them.  An event that fails at
+ * admission (device set-up) is finished with that error in its result's
+ * status; the other events go on. */
 int gvh_batch_submit(gvh_batch *b, const gvh_event *ev, long long *id);
 /* Admit queued events that fit, advance every running event by one step,
  * search due freeze-out windows, retire finished events.  *nrunning =

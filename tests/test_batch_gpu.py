@@ -237,3 +237,81 @@ def test_batch_argument_errors():
     with pytest.raises(gpuvh.GvhError):
         gpuvh._check(gpuvh.lib().gvh_batch_field(batch.h, 0, gpuvh.E, None))  # no such event
     batch.close()
+
+
+def test_invalid_submission_between_valid_events(monkeypatch):
+    """Submissions are checked as gvh_create checks a context (positive
+    spacings and eta/s): an invalid event between two valid ones is refused
+    at submit and the valid ones run to their ends, equal to their batches
+    of one."""
+    monkeypatch.setenv("GVH_VARIANT", "0")
+    table = gpuvh.EosTable()
+    opts = gpuvh.options(gpuvh.EOS_TABLE, True, gpuvh.FP64, 0.16 / gpuvh.HBARC, table=table)
+    events = make_events([(21, 21, 1, 0.2, 0.1, 0.02, 9, 31), (17, 19, 1, 0.2, 0.1, 0.02, 7, 32)],
+                         table)
+    batch = gpuvh.Batch(opts, max_events=4, autotune=False)
+    a = batch.submit(*events[0], keep_state=True)
+    L, H, e = events[1]
+    bad_h = gpuvh.hydro(0.5, 0.0)  # eta/s = 0
+    with pytest.raises(gpuvh.GvhError):
+        batch.submit(L, bad_h, e)
+    bad_l = gpuvh.lattice(17, 19, 1, 0.0, 0.2, 0.1, 0.02, nt=7)  # dx = 0
+    with pytest.raises(gpuvh.GvhError):
+        batch.submit(bad_l, H, e)
+    with pytest.raises(ValueError):
+        batch.submit(L, H, e[:-1])  # fewer values than the lattice
+    c = batch.submit(L, H, e, keep_state=True)
+    batch.run()
+    res = {r["id"]: r for r in batch.poll()}
+    assert set(res) == {a, c}
+    for i, ev in ((a, events[0]), (c, events[1])):
+        assert res[i]["status"] == gpuvh.OK
+        alone = run_batch(opts, [ev], False)[0]
+        assert res[i]["steps"] == alone[0]["steps"] == ev[0].numProperTimePoints + 1
+        for f in FIELDS:
+            np.testing.assert_array_equal(batch.field(i, f), alone[2][f])
+    batch.close()
+
+
+def test_newton_budget_retires_only_its_event(monkeypatch):
+    """An event submitted with a Newton iteration budget of 2 has root-finds
+    that need more (half-site states a few percent from the cell's own e),
+    so it is retired with status ENOCONV after its first step; the events
+    beside it run to their ends, bitwise equal to their batches of one.  The
+    same event with a budget it never reaches (1000) ends normally, equal to
+    its run without a budget."""
+    monkeypatch.setenv("GVH_VARIANT", "0")
+    table = gpuvh.EosTable()
+    opts = gpuvh.options(gpuvh.EOS_TABLE, True, gpuvh.FP64, 0.16 / gpuvh.HBARC, table=table)
+    specs = [(25, 25, 1, 0.2, 0.1, 0.02, 12, 41), (21, 23, 1, 0.2, 0.1, 0.02, 15, 42),
+             (13, 11, 7, 0.3, 0.3, 0.03, 9, 43)]
+    events = make_events(specs, table)
+    batch = gpuvh.Batch(opts, max_events=4, autotune=False)
+    ids = [batch.submit(L, H, e, keep_state=True, newton_budget=(2 if k == 1 else 0))
+           for k, (L, H, e) in enumerate(events)]
+    batch.run()
+    res = {r["id"]: r for r in batch.poll()}
+    say("  results", [(res[i]["steps"], res[i]["status"], res[i]["newton_not_converged"])
+                      for i in ids])
+    bad = res[ids[1]]
+    assert bad["status"] == gpuvh.ENOCONV
+    assert bad["newton_not_converged"] > 0 and bad["steps"] <= 2
+    for k in (0, 2):
+        r = res[ids[k]]
+        assert r["status"] == gpuvh.OK and r["newton_not_converged"] == 0
+        alone = run_batch(opts, [events[k]], False)[0]
+        assert r["steps"] == alone[0]["steps"]
+        for f in FIELDS:
+            np.testing.assert_array_equal(batch.field(ids[k], f), alone[2][f])
+    batch.close()
+    # a budget that is never reached changes nothing
+    batch = gpuvh.Batch(opts, max_events=2, autotune=False)
+    i = batch.submit(*events[1], keep_state=True, newton_budget=1000)
+    batch.run()
+    r = batch.poll()[0]
+    assert r["status"] == gpuvh.OK and r["newton_not_converged"] == 0
+    alone = run_batch(opts, [events[1]], False)[0]
+    assert r["steps"] == alone[0]["steps"]
+    for f in FIELDS:
+        np.testing.assert_array_equal(batch.field(i, f), alone[2][f])
+    batch.close()
