@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gpuvh.h"
@@ -858,5 +859,101 @@ extern "C" int gvh_cornelius_cells(int dim, long long ncells, const double *corn
         normals[(i * corn::MAXE + k) * dim + j] = el[(i * corn::MAXE + k) * ELEM + 4 + off + j];
       }
   if (ov) return gvh_fail(GVH_EINVAL, "%u cells exceed %d elements", ov, corn::MAXE);
+  return GVH_OK;
+}
+
+// ------------------------------------------- cornelius without allocation
+// gvh_cornelius_cells_noalloc: the same kernel on device buffers of the
+// caller, with its element records and overflow counter in a static device
+// array instead of hipMalloc'd memory.  Each GPU holds its own copy of the
+// array for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+
+namespace {
+
+constexpr long long NA_MAX_CELLS = GVH_CORNELIUS_NOALLOC_MAX_CELLS;
+constexpr long long NA_ELEM_DOUBLES = NA_MAX_CELLS * corn::MAXE * ELEM;
+// element records, then one 64-bit word holding the overflow counter (the
+// tail is padded to 256 bytes)
+static __device__ __align__(256) double scratch[NA_ELEM_DOUBLES + 32];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+double *na_scratch[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int noalloc_scratch(double **out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEVICES)
+    return gvh_fail(GVH_EINVAL, "gvh_cornelius_cells_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(na_mutex);
+  if (!na_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return gvh_fail(GVH_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+    na_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = na_scratch[dev];
+  return GVH_OK;
+}
+
+// Element records -> the caller's centroid and normal arrays ([cell][10][dim]),
+// one thread per (cell, element slot); slots past the cell's count are left
+// untouched, as gvh_cornelius_cells leaves them.
+__global__ __launch_bounds__(256) void cornelius_unpack_kernel(long long ncells, int dim,
+                                                               const int *__restrict__ counts,
+                                                               const double *__restrict__ elem,
+                                                               double *__restrict__ cen,
+                                                               double *__restrict__ nrm) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * corn::MAXE) return;
+  const long long cell = t / corn::MAXE;
+  const int k = (int)(t - cell * corn::MAXE);
+  if (k >= counts[cell]) return;
+  const int off = dim == 4 ? 0 : 1;
+  for (int j = 0; j < dim; ++j) {
+    cen[t * dim + j] = elem[t * ELEM + off + j];
+    nrm[t * dim + j] = elem[t * ELEM + 4 + off + j];
+  }
+}
+
+}  // namespace
+
+extern "C" int gvh_cornelius_cells_noalloc(int dim, long long ncells, const double *corners,
+                                           const double *dx, double v0, int *counts,
+                                           double *centroids, double *normals, void *stream) {
+  if (dim != 3 && dim != 4) return gvh_fail(GVH_EINVAL, "dim must be 3 or 4");
+  if (ncells < 0 || (ncells > 0 && (!corners || !dx || !counts || !centroids || !normals)))
+    return gvh_fail(GVH_EINVAL, "gvh_cornelius_cells_noalloc: NULL argument");
+  if (ncells > NA_MAX_CELLS)
+    return gvh_fail(GVH_EINVAL, "gvh_cornelius_cells_noalloc: %lld cells, at most %lld per call",
+                    ncells, NA_MAX_CELLS);
+  if (ncells == 0) return GVH_OK;
+  double *base = nullptr;
+  if (int rc = noalloc_scratch(&base)) return rc;
+  double *elem = base;
+  unsigned long long *ov_word = reinterpret_cast<unsigned long long *>(base + NA_ELEM_DOUBLES);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the counter keeps the last call's value: clear it in the stream
+  HIP_TRY(launch_fill_u64(ov_word, 1, 0ull, st));
+  CellSource S{};
+  S.window = false;
+  S.corners = corners;
+  hipLaunchKernelGGL(fo_cornelius_kernel, dim3((unsigned)ncells), dim3(64), 0, st, S, ncells, dim,
+                     dx[0], dx[1], dx[2], dim == 4 ? dx[3] : 0.0, v0, counts, elem,
+                     reinterpret_cast<unsigned *>(ov_word));
+  HIP_TRY(hipGetLastError());
+  const long long nslots = ncells * corn::MAXE;
+  hipLaunchKernelGGL(cornelius_unpack_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0,
+                     st, ncells, dim, counts, elem, centroids, normals);
+  HIP_TRY(hipGetLastError());
+  unsigned long long ov = 0;
+  HIP_TRY(hipMemcpyAsync(&ov, ov_word, sizeof ov, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (ov) return gvh_fail(GVH_EINVAL, "%llu cells exceed %d elements", ov, corn::MAXE);
   return GVH_OK;
 }

@@ -49,6 +49,8 @@ int fo_find_count(gvh_context *c, int n, hipStream_t st);
 int fo_find_elements(gvh_context *c, hipStream_t st);
 int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements);
 int fo_rows_async(gvh_context *c, double *rows, long long nrows, hipStream_t st);
+// Fill n 64-bit words in a stream with a kernel (no hipMemsetAsync node).
+hipError_t launch_fill_u64(unsigned long long *p, int n, unsigned long long v, hipStream_t st);
 }  // namespace gvh
 
 #define HIP_TRY(expr)                                                              \

@@ -173,6 +173,8 @@ def lib():
         "gvh_freezeout_slot": [C.c_void_p, C.c_int, C.c_int, dp],
         "gvh_cornelius_cells": [C.c_int, C.c_longlong, dp, dp, C.c_double, C.POINTER(C.c_int),
                                 dp, dp],
+        "gvh_cornelius_cells_noalloc": [C.c_int, C.c_longlong, C.c_void_p, dp, C.c_double,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
         "gvh_run": [C.POINTER(LatticeParameters), C.POINTER(InitialConditionParameters),
                     C.POINTER(HydroParameters), C.c_char_p, C.c_char_p, C.POINTER(Options),
                     C.POINTER(RunOptions), C.POINTER(RunStats)],

@@ -230,6 +230,21 @@ int gvh_freezeout_slot(gvh_context *ctx, int logical, int var, double *host_out)
  * normals [ncells][10][dim]. */
 int gvh_cornelius_cells(int dim, long long ncells, const double *corners, const double *dx,
                         double v0, int *counts, double *centroids, double *normals);
+/* gvh_cornelius_cells without allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that a HIP graph may capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  corners, counts, centroids and normals are DEVICE pointers in
+ * gvh_cornelius_cells' layouts; dx (dim values) is a host array.  The work
+ * runs on `stream` (a hipStream_t; NULL: the null stream) and the call
+ * returns once it has finished.  Its scratch (the element records and an
+ * overflow counter) is a static device array of about 10 MiB that each GPU
+ * holds until the process ends, so one call takes at most
+ * GVH_CORNELIUS_NOALLOC_MAX_CELLS cells (more: GVH_EINVAL).  Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU race. */
+#define GVH_CORNELIUS_NOALLOC_MAX_CELLS 16384
+int gvh_cornelius_cells_noalloc(int dim, long long ncells, const double *corners,
+                                const double *dx, double v0, int *counts, double *centroids,
+                                double *normals, void *stream);
 
 /* ------------------------------------------------- validity diagnostics
  * Knudsen and inverse Reynolds numbers per cell: checkValidityKernel
