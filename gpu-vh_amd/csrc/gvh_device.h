@@ -144,20 +144,37 @@ struct BatchSlot {
   long long nint;
 };
 
+// ------------------------------------------------------- cell access
+// Element s of an array with a uniform (scalar) base: the byte offset is
+// formed as a 32-bit product, so a load or store is the scalar-base +
+// 32-bit-vector-offset form and needs no 64-bit address per access.  Every
+// array of a lattice is below 4 GiB (the context checks the ghosted size).
+template <typename T>
+__device__ __forceinline__ T ld_cell(const T *base, unsigned s) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      s * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_cell(T *base, unsigned s, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + s * (unsigned)sizeof(T)) = v;
+}
+
 // ------------------------------------------------------------ limiter
 // approximateDerivative (FluxLimiter.cu:33-39): the nested generalized
 // minmod(theta*dl, minmod(dc, theta*dr)) with sgn(0)=+1 equals
 // sign * min(|theta dl|, |dc|, |theta dr|) when all three arguments are in the
 // same sign class (x < 0 versus x >= 0) and +0 otherwise -- exactly, since
 // min and the factors 2 and 1/2 do not round.  That form needs no products
-// of signs.
+// of signs.  With a = y - x, c = z - y, b = z - x: min(2|a|, |b|/2, 2|c|)
+// = 2 min(|a|, |c|, |b|/4) exactly, and a, c in one class force b into it
+// (y >= x and z >= y give z >= x, likewise for <), so the class test needs
+// only a and c.
 template <typename R>
 __device__ __forceinline__ R slope_t(R x, R y, R z) {
-  const R dl = R(THETA) * (y - x), dc = (z - x) * R(0.5), dr = R(THETA) * (z - y);
-  const R m = fmin(fabs(dl), fmin(fabs(dc), fabs(dr)));
-  const bool nl = dl < R(0), nc = dc < R(0), nr = dr < R(0);
-  const bool same = (nl == nc) & (nc == nr);
-  return same ? (nl ? -m : m) : R(0);
+  const R a = y - x, b = z - x, c = z - y;
+  const R m = R(THETA) * fmin(fmin(fabs(a), fabs(c)), fabs(b) * R(0.25));
+  const bool na = a < R(0), nc = c < R(0);
+  return na == nc ? (na ? -m : m) : R(0);
 }
 __device__ __forceinline__ double slope(double x, double y, double z) {
   return slope_t<double>(x, y, z);
@@ -602,7 +619,7 @@ __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double
   }
   if (ePrev <= 0.1) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) e[k] = M0[k] - M[k] / M0[k];
+    for (int k = 0; k < N; ++k) e[k] = M0[k] - div_t(M[k], M0[k]);
   } else {
     solve_e_lockstep<EOS, N, F>(T, F(ePrev), M0, M, Pi, e, diag, max_iters);
   }
@@ -622,7 +639,7 @@ __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double
       atomicAdd(&diag[DIAG_E_CLAMPED], 1ull);
     }
     const F P = p + Pi[k];
-    v[k] = (double)(((ek + P > F(0)) ? Md[k] : -Md[k]) / fabs(M0[k] + P));
+    v[k] = (double)div_t((ek + P > F(0)) ? Md[k] : -Md[k], fabs(M0[k] + P));
   }
 }
 

@@ -286,9 +286,9 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
   const R hc = R(A.half_inv_h[DIR]);
   // conserved variable n at cell s + o (n < 4: T^{tau mu}, else pi / Pi)
   auto ld = [&](int n, long long o) -> R {
-    return n < 4 ? R((qT + n * G + o)[s]) : R((qP + (n - 4) * G + o)[s]);
+    return n < 4 ? R(ld_cell(qT + n * G + o, s)) : R(ld_cell(qP + (n - 4) * G + o, s));
   };
-  auto ldp = [&](int n, long long o) -> R { return R((qP + (n - 4) * G + o)[s]); };
+  auto ldp = [&](int n, long long o) -> R { return R(ld_cell(qP + (n - 4) * G + o, s)); };
 
   // the six spatial pi components, whose flux comes after that of the kept
   // states: their 30 stencil values in one batch.  In FP32 they are loaded
@@ -569,14 +569,15 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   // conserved variable n of a buffer at this cell: T^{tau mu} in FP64, pi
   // and Pi in the storage type PS
   auto q_at = [&](const double *base, int n) -> double {
-    return n < 4 ? (base + n * G)[s]
-                 : (double)(reinterpret_cast<const PS *>(base + 4 * G) + (n - 4) * G)[s];
+    return n < 4 ? ld_cell(base + n * G, s)
+                 : (double)ld_cell(reinterpret_cast<const PS *>(base + 4 * G) + (n - 4) * G, s);
   };
   const double *u = A.uin;
-  const double ut = u[s], ux = (u + G)[s], uy = (u + 2 * G)[s], un = (u + 3 * G)[s];
-  const double e_s = A.ein[s];
+  const double ut = ld_cell(u, s), ux = ld_cell(u + G, s), uy = ld_cell(u + 2 * G, s),
+               un = ld_cell(u + 3 * G, s);
+  const double e_s = ld_cell(A.ein, s);
   if constexpr (FUSED) {
-    const double p_s = A.pin[s];
+    const double p_s = ld_cell(A.pin, s);
     GradT<R> g;
     const long long st[3] = {1, sy, sz};
 #pragma unroll
@@ -587,11 +588,11 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
       }
       const R ih = R(A.inv_h[d]);
       const double *um = u - st[d], *upl = u + st[d];
-      g.ut[d] = slope_t<R>(R(um[s]), R(ut), R(upl[s])) * ih;
-      g.ux[d] = slope_t<R>(R((um + G)[s]), R(ux), R((upl + G)[s])) * ih;
-      g.uy[d] = slope_t<R>(R((um + 2 * G)[s]), R(uy), R((upl + 2 * G)[s])) * ih;
-      g.un[d] = slope_t<R>(R((um + 3 * G)[s]), R(un), R((upl + 3 * G)[s])) * ih;
-      g.p[d] = slope_t<R>(R((A.pin - st[d])[s]), R(p_s), R((A.pin + st[d])[s])) * ih;
+      g.ut[d] = slope_t<R>(R(ld_cell(um, s)), R(ut), R(ld_cell(upl, s))) * ih;
+      g.ux[d] = slope_t<R>(R(ld_cell(um + G, s)), R(ux), R(ld_cell(upl + G, s))) * ih;
+      g.uy[d] = slope_t<R>(R(ld_cell(um + 2 * G, s)), R(uy), R(ld_cell(upl + 2 * G, s))) * ih;
+      g.un[d] = slope_t<R>(R(ld_cell(um + 3 * G, s)), R(un), R(ld_cell(upl + 3 * G, s))) * ih;
+      g.p[d] = slope_t<R>(R(ld_cell(A.pin - st[d], s)), R(p_s), R(ld_cell(A.pin + st[d], s))) * ih;
     }
     double Qd[15];
     R Q[15], S[15];
@@ -602,13 +603,14 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
     }
     S[14] = R(0);
     const double *up = A.uprev;
-    source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un), R(up[s]),
-                               R((up + G)[s]), R((up + 2 * G)[s]), R((up + 3 * G)[s]), g, S);
+    source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un),
+                               R(ld_cell(up, s)), R(ld_cell(up + G, s)),
+                               R(ld_cell(up + 2 * G, s)), R(ld_cell(up + 3 * G, s)), g, S);
 #pragma unroll
     for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Qd[n] + A.dt * (double)S[n];
   } else {
 #pragma unroll
-    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = (A.qout + n * G)[s];
+    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = ld_cell(A.qout + n * G, s);
   }
 
   // flux velocity of the cell itself for the directional sources (v = u/u^tau)
@@ -667,7 +669,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   }
 
   if constexpr (STAGE2) {
-    if (A.reg) A.reg[s] = fs_rec;
+    if (A.reg) st_cell(A.reg, s, fs_rec);
   }
 
   // write the cell and, on the boundary, its two ghost layers
@@ -677,13 +679,13 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   const bool gz0 = k == NG, gz1 = k == A.nz + 1;
   const long long sy2 = 2 * sy, sz2 = 2 * sz;
   auto put = [&](auto *base, auto val) {
-    base[s] = val;
-    if (gx0) { base[(long long)s - 1] = val; base[(long long)s - 2] = val; }
-    if (gx1) { base[(long long)s + 1] = val; base[(long long)s + 2] = val; }
-    if (gy0) { base[(long long)s - sy] = val; base[(long long)s - sy2] = val; }
-    if (gy1) { base[(long long)s + sy] = val; base[(long long)s + sy2] = val; }
-    if (gz0) { base[(long long)s - sz] = val; base[(long long)s - sz2] = val; }
-    if (gz1) { base[(long long)s + sz] = val; base[(long long)s + sz2] = val; }
+    st_cell(base, s, val);
+    if (gx0) { st_cell(base - 1, s, val); st_cell(base - 2, s, val); }
+    if (gx1) { st_cell(base + 1, s, val); st_cell(base + 2, s, val); }
+    if (gy0) { st_cell(base - sy, s, val); st_cell(base - sy2, s, val); }
+    if (gy1) { st_cell(base + sy, s, val); st_cell(base + sy2, s, val); }
+    if (gz0) { st_cell(base - sz, s, val); st_cell(base - sz2, s, val); }
+    if (gz1) { st_cell(base + sz, s, val); st_cell(base + sz2, s, val); }
   };
   PS *qoutP = reinterpret_cast<PS *>(A.qout + 4 * G);
 #pragma unroll 1
