@@ -152,7 +152,10 @@ static int check_dims(const gvh_lattice_parameters *L) {
                     L->numLatticePointsX, L->numLatticePointsY, L->numLatticePointsRapidity);
   const double G = (double)(L->numLatticePointsX + 4) * (L->numLatticePointsY + 4) *
                    (L->numLatticePointsRapidity + 4);
-  if (G > 2.0e9) return gvh_fail(GVH_EINVAL, "lattice too large for 32-bit cell ids");
+  // the kernels address a cell of one array by a 32-bit byte offset
+  if (G * 8.0 >= 4294967296.0)
+    return gvh_fail(GVH_EINVAL, "lattice too large: %.0f ghosted cells, at most 2^29 (an array "
+                    "of doubles must stay below 4 GiB)", G);
   if (!(L->latticeSpacingX > 0 && L->latticeSpacingY > 0 && L->latticeSpacingRapidity > 0 &&
         L->latticeSpacingProperTime > 0))
     return gvh_fail(GVH_EINVAL, "lattice spacings must be positive");
@@ -619,6 +622,51 @@ extern "C" int gvh_steps(gvh_context *c, double t0, int n_begin, int nsteps) {
     int rc = step_impl(c, t0 + n * dt, nullptr, nullptr);
     if (rc) return rc;
   }
+  return GVH_OK;
+}
+
+// nsteps steps with an event before the first stage and after every stage,
+// one wait at the end: the stage kernel times of exactly the steps that the
+// caller's clock brackets.
+extern "C" int gvh_steps_timed(gvh_context *c, double t0, int n_begin, int nsteps,
+                               float *ms_stage1, float *ms_stage2, float *ms_total) {
+  if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  if (!c->initialized) return gvh_fail(GVH_EINVAL, "state not initialised (gvh_set_initial_state)");
+  if (nsteps < 1) return gvh_fail(GVH_EINVAL, "nsteps must be >= 1");
+  const double dt = c->L.latticeSpacingProperTime;
+  std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 1, nullptr);
+  int rc = GVH_OK;
+  hipError_t e = hipSuccess;
+  for (auto &x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], c->stream);
+  for (int k = 0; k < nsteps && e == hipSuccess && !rc; ++k) {
+    const double t = t0 + (n_begin + k) * dt;
+    StageArgs A1 = make_stage_args(c, 1, t);
+    if ((rc = launch_stage(c, A1, false))) break;
+    if ((e = hipEventRecord(ev[2 * k + 1], c->stream)) != hipSuccess) break;
+    StageArgs A2 = make_stage_args(c, 2, t);
+    if ((rc = launch_stage(c, A2, true))) break;
+    if ((e = hipEventRecord(ev[2 * k + 2], c->stream)) != hipSuccess) break;
+    rotate_after_step(c);
+  }
+  float s1 = 0, s2 = 0, tot = 0;
+  if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[2 * nsteps]);
+  for (int k = 0; k < nsteps && !rc && e == hipSuccess; ++k) {
+    float a = 0, b = 0;
+    e = hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]);
+    s1 += a;
+    s2 += b;
+  }
+  if (!rc && e == hipSuccess) e = hipEventElapsedTime(&tot, ev[0], ev[2 * nsteps]);
+  for (auto &x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (rc) return rc;
+  if (e != hipSuccess) return gvh_fail(GVH_EHIP, "gvh_steps_timed: %s", hipGetErrorString(e));
+  if (ms_stage1) *ms_stage1 = s1;
+  if (ms_stage2) *ms_stage2 = s2;
+  if (ms_total) *ms_total = tot;
   return GVH_OK;
 }
 

@@ -156,6 +156,8 @@ def lib():
         "gvh_step": [C.c_void_p, C.c_double],
         "gvh_steps": [C.c_void_p, C.c_double, C.c_int, C.c_int],
         "gvh_step_timed": [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "gvh_steps_timed": [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_float),
+                            C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "gvh_synchronize": [C.c_void_p],
         "gvh_get_field": [C.c_void_p, C.c_int, C.c_int, dp],
         "gvh_count_above": [C.c_void_p, C.c_double, C.POINTER(C.c_longlong)],
@@ -366,6 +368,14 @@ class Context:
         a, b = C.c_float(), C.c_float()
         _check(lib().gvh_step_timed(self.h, t, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def steps_timed(self, t0, n_begin, nsteps):
+        """nsteps steps with HIP events around every stage kernel; returns the
+        summed stage-1 and stage-2 kernel ms and the GPU ms of the sequence."""
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        _check(lib().gvh_steps_timed(self.h, t0, n_begin, nsteps, C.byref(a), C.byref(b),
+                                     C.byref(c)))
+        return a.value, b.value, c.value
 
     def synchronize(self):
         _check(lib().gvh_synchronize(self.h))

@@ -162,6 +162,14 @@ int gvh_steps(gvh_context *ctx, double t0, int n_begin, int nsteps);
 /* One step with a HIP event pair around each of the two stage kernels;
  * blocks until done and returns their durations in milliseconds. */
 int gvh_step_timed(gvh_context *ctx, double t, float *ms_stage1, float *ms_stage2);
+/* nsteps steps from run-loop index n_begin (as gvh_steps) with HIP events on
+ * the context's stream around every stage kernel, waited for once at the end:
+ * the summed stage-1 and stage-2 kernel times and the GPU time of the whole
+ * sequence, in ms.  A measurement helper with no counterpart in the
+ * reference (its only timer is clock() around the step,
+ * HydroPlugin.cpp:88-105,414-426). */
+int gvh_steps_timed(gvh_context *ctx, double t0, int n_begin, int nsteps, float *ms_stage1,
+                    float *ms_stage2, float *ms_total);
 int gvh_synchronize(gvh_context *ctx);
 
 /* Copy a field of the current state to the host: interior only
