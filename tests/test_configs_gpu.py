@@ -1,0 +1,345 @@
+"""The benchmark configurations that take many steps or many events, on the
+device against the oracle (the FP64 CPU restatement of the reference KT step):
+
+- batched 2+1D events (the 64-event workload: transverse grids 121^2..261^2,
+  300..800 steps, 50..300 hotspots, tabulated EoS, shear + bulk): three
+  events spanning that range in one autotuned batch, in FP64 and in the mixed
+  form, against the oracle run one by one;
+- the mixed form on the 201x201x101 event: the state after the steps of the
+  first freeze-out window and that window's surface at T_fo = 0.15 GeV,
+  against the oracle (state) and the FP64 form checked against the oracle in
+  the same test (surface);
+- the serving loop: events of the serving workload arriving at Poisson times
+  while others run, each result equal to the same event run alone.
+
+Mixed-form errors are measured where the north-star criterion puts them, on
+the fluid at or above the freeze-out energy density e_fo (T >= 0.15 GeV):
+e relative per cell, u^mu as max_mu |du^mu| / u^tau (u^eta scaled by tau),
+pi^{mu nu} relative to the largest |pi| there.
+"""
+import numpy as np
+import pytest
+
+import gpuvh
+import oracle_lib as O
+from gpuvh import ic
+from parity import PI_FIELDS, U_FIELDS, floor_scales, rel_err
+
+pytestmark = pytest.mark.gpu
+
+HBARC = gpuvh.HBARC
+T_FO = 0.15  # GeV
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def table_opts(table, precision):
+    return gpuvh.options(gpuvh.EOS_TABLE, True, precision, 0.16 / HBARC, table=table)
+
+
+def oracle_for(nx, ny, nz, dt, table, dz=0.1):
+    P = O.make_params(nx, ny, nz, 0.1, 0.1, dz, dt, 0.16, 1, O.EOS_TABLE,
+                      zeta_tpeak=0.16 / HBARC, table=table)
+    return O.Oracle(P)
+
+
+def e_fo_of(table):
+    """Energy density at T_fo on the table, as the run loop computes it
+    (equilibriumEnergyDensity(T_fo / hbarc), HydroPlugin.cpp:132-135)."""
+    L = gpuvh.lattice(5, 5, 1, 0.1, 0.1, 0.1, 0.01)
+    ctx = gpuvh.Context(L, gpuvh.hydro(0.5, 0.16, T_fo_GeV=T_FO), table_opts(table, gpuvh.FP64))
+    e_fo = ctx.energy_density_of_T(T_FO / HBARC)
+    ctx.close()
+    return e_fo
+
+
+def mixed_errors(ref, got, tau, e_fo):
+    """Errors of the mixed form on the cells with e_ref >= e_fo; ref, got:
+    {field: array}."""
+    m = ref[gpuvh.E] >= e_fo
+    out = {"cells": int(m.sum())}
+    if not m.any():
+        return out
+    e_r = ref[gpuvh.E][m]
+    out["e"] = float(np.max(np.abs(got[gpuvh.E][m] - e_r) / e_r))
+    ut = ref[gpuvh.UT][m]
+    du = 0.0
+    for f in U_FIELDS:
+        if f not in ref:
+            continue
+        s = tau if f == gpuvh.UN else 1.0
+        du = max(du, float(np.max(np.abs(got[f][m] - ref[f][m]) * s / ut)))
+    out["u"] = du
+    pis = [f for f in PI_FIELDS if f in ref]
+    if pis:
+        scale = max(float(np.max(np.abs(ref[f][m]))) for f in pis)
+        if scale > 0:
+            out["pi"] = max(float(np.max(np.abs(got[f][m] - ref[f][m]))) for f in pis) / scale
+    return out
+
+
+# ------------------------------------------------------- batched events
+C4_FIELDS = [gpuvh.E, gpuvh.UT, gpuvh.UX, gpuvh.UY, gpuvh.PITT, gpuvh.PITX, gpuvh.PITY,
+             gpuvh.PIXX, gpuvh.PIXY, gpuvh.PIYY, gpuvh.PINN, gpuvh.PI]
+# (transverse points, steps nt: the run takes nt + 1, seed, hotspots)
+C4_EVENTS = [(121, 300, 101, 50), (191, 550, 102, 175), (261, 800, 103, 300)]
+
+
+@pytest.fixture(scope="module")
+def c4_oracle():
+    """The three events on the oracle, one by one, in FP64; and each again
+    from an initial e one ulp larger in every cell, which measures how far
+    rounding-level differences grow over the event's steps."""
+    table = gpuvh.EosTable()
+    out = []
+    for nx, nt, seed, nh in C4_EVENTS:
+        e = ic.serving_event(nx, nx, 1, 0.1, 0.1, seed, nh)
+        runs = []
+        for e_init in (e, np.nextafter(e, np.inf)):
+            orc = oracle_for(nx, nx, 1, 0.01, table)
+            ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+            orc.set_primitives(0.5, e_init, O.eos_p_batch(orc.P, e_init), ut, ux, uy, un, pi10, Pi)
+            say("oracle event %d^2, %d steps" % (nx, nt + 1))
+            orc.steps(0.5, 0, nt + 1)
+            runs.append({f: orc.get(f) for f in C4_FIELDS})
+            orc.close()
+        out.append((e, runs[0], 0.5 + (nt + 1) * 0.01, runs[1]))
+    return table, out
+
+
+def run_c4_batch(table, precision, events):
+    b = gpuvh.Batch(table_opts(table, precision), max_events=len(events), autotune=True)
+    ids = []
+    for (nx, nt, seed, nh), e in zip(C4_EVENTS, events):
+        L = gpuvh.lattice(nx, nx, 1, 0.1, 0.1, 0.1, 0.01, nt=nt)
+        H = gpuvh.hydro(0.5, 0.16, T_fo_GeV=0.0)  # fixed step count, never frozen
+        ids.append(b.submit(L, H, e, keep_state=True))
+    b.run()
+    res = {r["id"]: r for r in b.poll()}
+    fields = []
+    for (nx, nt, seed, nh), i in zip(C4_EVENTS, ids):
+        assert res[i]["status"] == 0 and res[i]["steps"] == nt + 1, res[i]
+        fields.append({f: b.field(i, f) for f in C4_FIELDS})
+    b.close()
+    return fields
+
+
+def test_batched_events_fp64_against_oracle(c4_oracle):
+    """FP64 batch of 121^2 x 301, 191^2 x 551 and 261^2 x 801 steps against
+    the oracle.  e, u^tau and Pi (which never cross zero) within 1e-10
+    relative per cell.  Every field's largest deviation, as a fraction of its
+    tensor's scale, within 4 times the oracle's own deviation from a run
+    whose initial e is one ulp larger: after hundreds of steps the rounding
+    differences of the two builds (FMA contraction on the device, none in
+    the oracle) have grown exactly as a one-ulp perturbation grows, to
+    ~1e-12 of the scale, which in components that cross zero (u^x, pi^{xy},
+    ...) is more than 1e-10 of their local value."""
+    table, ref = c4_oracle
+    got = run_c4_batch(table, gpuvh.FP64, [r[0] for r in ref])
+    bad = []
+    for (nx, nt, _, _), (e, rf, tau, rf1), g in zip(C4_EVENTS, ref, got):
+        scales = floor_scales(rf, tau)
+        strict = {gpuvh.FIELD_NAMES[f]: rel_err(g[f], rf[f], scale=scales[f])
+                  for f in (gpuvh.E, gpuvh.UT, gpuvh.PI)}
+        dev = {}
+        for f in C4_FIELDS:
+            if scales[f] == 0:
+                continue
+            d_dev = float(np.abs(g[f] - rf[f]).max() / scales[f])
+            d_ulp = float(np.abs(rf1[f] - rf[f]).max() / scales[f])
+            dev[gpuvh.FIELD_NAMES[f]] = (d_dev, d_ulp)
+        say("fp64 batch %d^2 x %d: strict %s" % (nx, nt + 1,
+                                                 {k: "%.1e" % v for k, v in strict.items()}))
+        say("  device / one-ulp deviation (fraction of scale):",
+            {k: "%.1e/%.1e" % v for k, v in dev.items()})
+        bad += ["%d^2 %s %.2e" % (nx, k, v) for k, v in strict.items() if v > 1e-10]
+        bad += ["%d^2 %s %.2e > 4 x %.2e" % (nx, k, a, b) for k, (a, b) in dev.items()
+                if a > 4 * b + 1e-15]
+    assert not bad, bad
+
+
+def test_batched_events_mixed_against_oracle(c4_oracle):
+    """Mixed-form batch of the same events against the FP64 oracle, on the
+    fluid at or above e_fo: e within 1e-4 relative per cell, u^mu within
+    1e-4 of u^tau; pi reported."""
+    table, ref = c4_oracle
+    got = run_c4_batch(table, gpuvh.MIXED, [r[0] for r in ref])
+    e_fo = e_fo_of(table)
+    for (nx, nt, _, _), (e, rf, tau, _), g in zip(C4_EVENTS, ref, got):
+        err = mixed_errors(rf, g, tau, e_fo)
+        say("mixed batch %d^2 x %d: %s" % (nx, nt + 1, err))
+        assert err["cells"] > 0
+        assert err["e"] <= 1e-4 and err["u"] <= 1e-4, err
+
+
+# --------------------------------------------- the 201x201x101 event, mixed
+FOFREQ = 10
+
+
+def test_mixed_full_event_state_and_surface():
+    """201x201x101 benchmark event (tabulated EoS, shear + bulk), first
+    freeze-out window at T_fo = 0.15 GeV (searched at run-loop index 9):
+
+    - the FP64 form equals the oracle per cell to 1e-10 after the window's
+      steps (so its window rows are the oracle's surface);
+    - the mixed form's e and u^mu on the fluid at or above e_fo are within
+      1e-4 of the oracle;
+    - the mixed window's elements (matched by cube and element) have normals
+      within 1e-4 of the largest normal, e within 1e-4 of e_fo and u^mu
+      within 1e-4 of u^tau at the centroids; the two surfaces share all but
+      a small fraction of their elements (cubes whose corners straddle e_fo
+      within the rounding of either form)."""
+    table = gpuvh.EosTable()
+    nx, ny, nz = 201, 201, 101
+    e = ic.c2_event(nx, ny, nz, 0.1, 0.1, 0.1, seed=2)
+    nsteps = FOFREQ
+    t0, dt = 0.5, 0.01
+    fields = [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI]
+    say("oracle 201x201x101, %d steps" % nsteps)
+    orc = oracle_for(nx, ny, nz, dt, table)
+    ut, ux, uy, un, pi10, Pi = ic.at_rest(e.shape)
+    p = O.eos_p_batch(orc.P, e)
+    orc.set_primitives(t0, e, p, ut, ux, uy, un, pi10, Pi)
+    orc.steps(t0, 0, nsteps)
+    ref = {f: orc.get(f) for f in fields}
+    orc.close()
+    tau = t0 + nsteps * dt
+    e_fo = e_fo_of(table)
+
+    L = gpuvh.lattice(nx, ny, nz, 0.1, 0.1, 0.1, dt)
+    H = gpuvh.hydro(t0, 0.16, T_fo_GeV=T_FO)
+    rows, state = {}, {}
+    for prec, name in ((gpuvh.FP64, "fp64"), (gpuvh.MIXED, "mixed")):
+        say("device", name)
+        ctx = gpuvh.Context(L, H, table_opts(table, prec))
+        ctx.set_initial_state(t0, e, ctx.equilibrium_pressure(e), ut, ux, uy, un, pi10, Pi)
+        ctx.freezeout_init(FOFREQ, e_fo, 0)
+        t = t0
+        for n in range(1, nsteps + 1):
+            ctx.freezeout_record(n)
+            if n % FOFREQ == FOFREQ - 1:
+                rows[name] = ctx.freezeout_find(n, t)
+            ctx.step(t)
+            t = t0 + n * dt
+        state[name] = {f: ctx.get(f) for f in fields}
+        assert ctx.diagnostics() == {"newton_not_converged": 0, "e_nan": 0, "e_clamped": 0}
+        ctx.close()
+
+    scales = floor_scales(ref, tau)
+    fp64_err = max(rel_err(state["fp64"][f], ref[f], scale=scales[f]) for f in fields)
+    say("fp64 vs oracle: %.2e" % fp64_err)
+    assert fp64_err <= 1e-10
+
+    err = mixed_errors(ref, state["mixed"], tau, e_fo)
+    say("mixed vs oracle on e >= e_fo:", err)
+    assert err["e"] <= 1e-4 and err["u"] <= 1e-4, err
+
+    a, b = rows["fp64"], rows["mixed"]
+    say("surface rows: fp64 %d, mixed %d" % (len(a), len(b)))
+    assert len(a) > 1000
+    ka, kb = element_keys(a, t0, dt), element_keys(b, t0, dt)
+    common = sorted(set(ka) & set(kb))
+    frac_unmatched = 1.0 - len(common) / max(len(ka), len(kb))
+    ia = np.array([ka[k] for k in common])
+    ib = np.array([kb[k] for k in common])
+    A, B = a[ia], b[ib]
+    nscale = np.abs(A[:, 4:8]).max()
+    d_norm = float(np.abs(B[:, 4:8] - A[:, 4:8]).max() / nscale)
+    d_e = float(np.abs(B[:, 11] - A[:, 11]).max() / e_fo)
+    # u^tau of the fp64 rows from normalisation (rows hold u^x, u^y, u^eta)
+    tau_c = A[:, 0]
+    utc = np.sqrt(1 + A[:, 8] ** 2 + A[:, 9] ** 2 + (tau_c * A[:, 10]) ** 2)
+    du = np.abs(B[:, 8:11] - A[:, 8:11])
+    du[:, 2] *= tau_c
+    d_u = float((du.max(axis=1) / utc).max())
+    say("surface: unmatched %.2e, normals %.2e, e %.2e, u %.2e" % (frac_unmatched, d_norm, d_e,
+                                                                  d_u))
+    assert frac_unmatched <= 1e-3
+    assert d_norm <= 1e-4 and d_e <= 1e-4 and d_u <= 1e-4
+
+
+def element_keys(rows, t0, dt):
+    """{(cube, element index within the cube): row} for window rows in the
+    reference's (it, ix, iy, ieta, element) order.  The cube of a row is the
+    lattice cell below its centroid (tau, x, y, eta)."""
+    out = {}
+    h = 0.1
+    cube = np.stack([np.floor((rows[:, 0] - t0) / dt + 1e-9),
+                     np.floor(rows[:, 1] / h + 1e-9), np.floor(rows[:, 2] / h + 1e-9),
+                     np.floor(rows[:, 3] / h + 1e-9)], axis=1).astype(np.int64)
+    prev, k = None, 0
+    for r, c in enumerate(map(tuple, cube)):
+        k = k + 1 if c == prev else 0
+        prev = c
+        out[(c, k)] = r
+    return out
+
+
+# ------------------------------------------------------------ serving
+def test_serving_poisson_arrivals_equal_alone():
+    """Six events of the serving workload (2+1D 121^2..161^2 and 3+1D
+    121^2 x 21..31, hotspots on a Gaussian base, tabulated EoS, shear + bulk,
+    through freeze-out at T_fo = 0.15 GeV with surfaces) submitted at
+    seeded Poisson times into a running batch, mixed form: every event's
+    steps, stop, surface rows and final fields equal the same event run in a
+    batch of its own, bit for bit."""
+    table = gpuvh.EosTable()
+    opts = table_opts(table, gpuvh.MIXED)
+    rng = np.random.default_rng(55)
+    specs = []
+    for k in range(6):
+        nx = 2 * int(rng.integers(60, 81)) + 1
+        nz = 1 if k % 2 == 0 else 2 * int(rng.integers(10, 16)) + 1
+        specs.append((nx, nz, 200 + k, int(rng.integers(50, 301))))
+    events = []
+    for nx, nz, seed, nh in specs:
+        e = ic.serving_event(nx, nx, nz, 0.1, 0.1, seed, nh)
+        L = gpuvh.lattice(nx, nx, nz, 0.1, 0.1, 0.1, 0.01, nt=4000)
+        H = gpuvh.hydro(0.5, 0.16, T_fo_GeV=T_FO)
+        events.append((L, H, e))
+    # arrivals in batch steps: exponential gaps of mean 40 steps
+    arrive = np.cumsum(rng.exponential(40.0, len(events))).astype(int)
+    fields = [gpuvh.E, gpuvh.UX, gpuvh.UY, gpuvh.PIXX, gpuvh.PI]
+
+    def finish(b, ids):
+        res = {r["id"]: r for r in b.poll()}
+        out = []
+        for i in ids:
+            r = res[i]
+            out.append((r["steps"], r["frozen"], r["status"], b.rows(i, r["surface_rows"]),
+                        {f: b.field(i, f) for f in fields}))
+        return out
+
+    b = gpuvh.Batch(opts, max_events=8, autotune=True)
+    ids, step, nsub, overlapped = [], 0, 0, 0
+    while True:
+        while nsub < len(events) and arrive[nsub] <= step:
+            L, H, e = events[nsub]
+            ids.append(b.submit(L, H, e, surface=True, keep_state=True))
+            nsub += 1
+        running = b.step()
+        overlapped = max(overlapped, running)
+        step += 1
+        if running == 0 and nsub == len(events):
+            break
+        assert step < 20000
+    say("serving: %d batch steps, up to %d events at once" % (step, overlapped))
+    assert overlapped >= 2
+    mix = finish(b, ids)
+    b.close()
+    for k, ev in enumerate(events):
+        b1 = gpuvh.Batch(opts, max_events=1, autotune=True)
+        i = b1.submit(*ev, surface=True, keep_state=True)
+        b1.run()
+        alone = finish(b1, [i])[0]
+        b1.close()
+        m = mix[k]
+        say("  event %d %s: steps %d frozen %d rows %d" % (k, specs[k][:2], m[0], m[1], len(m[3])))
+        assert m[2] == 0, "event %d ended with status %d" % (k, m[2])
+        assert m[0] == alone[0] and m[1] == alone[1]
+        np.testing.assert_array_equal(m[3], alone[3])
+        for f in fields:
+            np.testing.assert_array_equal(m[4][f], alone[4][f])
+

@@ -264,12 +264,46 @@ def read_snapshot(path):
     return np.loadtxt(path)
 
 
+def check_run_against_restatement(case, geom, Tfo, oracle_state, nt, stats, root, out):
+    """The run's steps, all-frozen stop, surface.dat and FREQ snapshots
+    against the restated run loop driven by the oracle from oracle_state
+    (surface text line for line, values that print differently within 1e-5
+    relative)."""
+    orc = case.oracle()
+    orc.set_primitives(case.t0, *(oracle_state(orc.P) if callable(oracle_state) else oracle_state))
+    e_fo = O.e_of_T(orc.P, Tfo / gpuvh.HBARC)
+    snap_fields = [O.E, O.UX, O.PIXX]
+    rows, steps, frozen, snaps = F.run_loop(orc, geom, nt, e_fo, oracle_eos_T_p(orc.P),
+                                            bool(case.bulk), snapshot_fields=snap_fields)
+    assert stats["steps"] == steps and bool(stats["frozen"]) == frozen
+    assert stats["surface_rows"] == len(rows)
+    lines, got = parse_surface(os.path.join(root, "output", "surface.dat"))
+    ref_lines = [" ".join("%g" % x for x in r) for r in rows]
+    assert len(lines) == len(ref_lines)
+    differ = [i for i, (a, b) in enumerate(zip(lines, ref_lines)) if a != b]
+    say("  rows", len(lines), "lines printed differently", len(differ))
+    for i in differ:
+        np.testing.assert_allclose(got[i], rows[i], rtol=1e-5, atol=1e-7 * np.abs(rows[i]).max())
+    assert len(differ) <= 0.01 * len(lines) + 1
+    # snapshots: file per field and FREQ time, %.8f values
+    nx, ny, nz = geom["nx"], geom["ny"], geom["nz"]
+    names = {O.E: "e", O.UX: "ux", O.PIXX: "pixx"}
+    for tsnap, fields in snaps.items():
+        for f, ref in fields.items():
+            path = os.path.join(out, "%s_%.3f.dat" % (names[f], tsnap))
+            d = read_snapshot(path)
+            assert d.shape == (nx * ny * nz, 4)
+            v = d[:, 3].reshape((nx, ny, nz), order="F")
+            np.testing.assert_allclose(v, ref, rtol=0, atol=2e-8 + 1e-10 * np.abs(ref).max(),
+                                       err_msg=path)
+    return len(rows)
+
+
 @pytest.mark.parametrize("ev", EVENTS, ids=[e[0] for e in EVENTS])
 def test_run_loop_matches_restatement(ev, tmp_path):
     """gvh_run on type-10 input files: steps taken, the all-frozen stop,
     surface.dat and the FREQ snapshots equal the restated run loop driven by
-    the oracle on the same files (surface text line for line, values that
-    print differently within 1e-5 relative)."""
+    the oracle on the same files."""
     case, geom, Tfo = event_case(ev)
     name, nx, ny, nz, dx, dz, dt = ev[:7]
     say("run loop", name)
@@ -286,34 +320,75 @@ def test_run_loop_matches_restatement(ev, tmp_path):
     o = gpuvh.options(case.eos, case.bulk, gpuvh.FP64, case.zeta_tpeak, table=case.table)
     stats = gpuvh.run(L, IC, H, root, out, o, gpuvh.run_options())
     say("  stats", stats)
-    # restatement on the oracle
-    orc = case.oracle()
     e_ic = np.float32(state[0] - 1e-3).astype(np.float64) + 1e-3
-    orc.set_primitives(case.t0, e_ic, O.eos_p_batch(orc.P, e_ic), *state[2:])
-    e_fo = O.e_of_T(orc.P, Tfo / gpuvh.HBARC)
-    snap_fields = [O.E, O.UX, O.PIXX]
-    rows, steps, frozen, snaps = F.run_loop(orc, geom, nt, e_fo, oracle_eos_T_p(orc.P),
-                                            bool(case.bulk), snapshot_fields=snap_fields)
-    assert stats["steps"] == steps and bool(stats["frozen"]) == frozen
-    assert stats["surface_rows"] == len(rows)
-    lines, got = parse_surface(os.path.join(root, "output", "surface.dat"))
-    ref_lines = [" ".join("%g" % x for x in r) for r in rows]
-    assert len(lines) == len(ref_lines)
-    differ = [i for i, (a, b) in enumerate(zip(lines, ref_lines)) if a != b]
-    say("  rows", len(lines), "lines printed differently", len(differ))
-    for i in differ:
-        np.testing.assert_allclose(got[i], rows[i], rtol=1e-5, atol=1e-7 * np.abs(rows[i]).max())
-    assert len(differ) <= 0.01 * len(lines) + 1
-    # snapshots: file per field and FREQ time, %.8f values
-    names = {O.E: "e", O.UX: "ux", O.PIXX: "pixx"}
-    for tsnap, fields in snaps.items():
-        for f, ref in fields.items():
-            path = os.path.join(out, "%s_%.3f.dat" % (names[f], tsnap))
-            d = read_snapshot(path)
-            assert d.shape == (nx * ny * nz, 4)
-            v = d[:, 3].reshape((nx, ny, nz), order="F")
-            np.testing.assert_allclose(v, ref, rtol=0, atol=2e-8 + 1e-10 * np.abs(ref).max(),
-                                       err_msg=path)
+    orc_state = lambda P: (e_ic, O.eos_p_batch(P, e_ic)) + tuple(state[2:])  # noqa: E731
+    check_run_against_restatement(case, geom, Tfo, orc_state, nt, stats, root, out)
+
+
+def write_tmunu_file(root, cols, nx, ny, nz, dx, dz):
+    """Type-11 input (InitialConditions.cpp:29-95): one row of 20 floats per
+    cell, x y z e p ut ux uy un pitt pitx pity pitn pixx pixy pixn piyy piyn
+    pinn Pi, x outermost and eta innermost; e holds e - 1e-3 (the reader adds
+    1e-3 back).  cols: the 17 value columns as float32 arrays."""
+    os.makedirs(os.path.join(root, "input"), exist_ok=True)
+    os.makedirs(os.path.join(root, "output"), exist_ok=True)
+    xs = (np.arange(nx) - (nx - 1) / 2.0) * dx
+    ys = (np.arange(ny) - (ny - 1) / 2.0) * dx
+    zs = (np.arange(nz) - (nz - 1) / 2.0) * dz
+    with open(os.path.join(root, "input", "Tmunu.dat"), "w") as f:
+        for i in range(nx):
+            for j in range(ny):
+                for k in range(nz):
+                    vals = " ".join("%.9g" % c[i, j, k] for c in cols)
+                    f.write("%.6f %.6f %.6f %s\n" % (xs[i], ys[j], zs[k], vals))
+
+
+@pytest.mark.parametrize("ev", [EVENTS[2], EVENTS[3]], ids=[EVENTS[2][0], EVENTS[3][0]])
+def test_run_loop_tmunu_file(ev, tmp_path):
+    """gvh_run on a type-11 Tmunu.dat (InitialConditions.cpp:29-95) with flow,
+    shear and bulk in every column: the run equals the restated run loop
+    driven by the oracle from the file's float values.  The file's p is 0.2%
+    above the EoS value, so a reader that recomputed p from e, dropped a
+    column or shifted the columns would fail the comparison."""
+    case, geom, Tfo = event_case(ev)
+    name, nx, ny, nz, dx, dz, dt = ev[:7]
+    say("run loop, Tmunu.dat", name)
+    e = smooth_state(case, 11)[0]
+    t0 = case.t0
+    X, Y, Z = np.meshgrid((np.arange(nx) - (nx - 1) / 2.0) * dx,
+                          (np.arange(ny) - (ny - 1) / 2.0) * dx,
+                          (np.arange(nz) - (nz - 1) / 2.0) * dz, indexing="ij")
+    ext = 0.5 * (nx - 1) * dx
+    ux = 0.08 * np.tanh(X / ext) + 0.02 * np.sin(Y)
+    uy = -0.06 * np.tanh(Y / ext)
+    un = (0.01 * np.tanh(Z) / t0) if nz > 1 else np.zeros_like(X)
+    ut = np.sqrt(1.0 + ux * ux + uy * uy + t0 * t0 * un * un)
+    p_eos = case.pressure(e)
+    p_file = 1.002 * p_eos
+    w = 0.01 * p_eos
+    pi10 = np.stack([0.3 * w, 0.1 * w * np.cos(X), -0.1 * w * np.sin(Y), 0.05 * w / t0,
+                     w * (1 + 0.2 * np.tanh(X)), 0.2 * w, 0.05 * w / t0, -0.5 * w,
+                     -0.04 * w / t0, -0.5 * w / (t0 * t0)])
+    Pi = -0.02 * p_eos if case.bulk else np.zeros_like(e)
+    cols = [np.float32(e - 1e-3), np.float32(p_file), np.float32(ut), np.float32(ux),
+            np.float32(uy), np.float32(un)] + [np.float32(c) for c in pi10] + [np.float32(Pi)]
+    root = str(tmp_path / "root")
+    out = str(tmp_path / "out")
+    os.makedirs(out)
+    write_tmunu_file(root, cols, nx, ny, nz, dx, dz)
+    nt = 120
+    L = gpuvh.lattice(nx, ny, nz, dx, dx, dz, dt, nt=nt)
+    IC = gpuvh.InitialConditionParameters()
+    IC.initialConditionType = 11
+    H = gpuvh.hydro(t0, 0.2, T_fo_GeV=Tfo)
+    o = gpuvh.options(case.eos, case.bulk, gpuvh.FP64, case.zeta_tpeak, table=case.table)
+    stats = gpuvh.run(L, IC, H, root, out, o, gpuvh.run_options())
+    say("  stats", stats)
+    f64 = [c.astype(np.float64) for c in cols]
+    orc_state = (f64[0] + 1e-3, f64[1], f64[2], f64[3], f64[4], f64[5],
+                 np.stack(f64[6:16]), f64[16] if case.bulk else np.zeros_like(e))
+    nrows = check_run_against_restatement(case, geom, Tfo, orc_state, nt, stats, root, out)
+    assert nrows > 0
 
 
 def test_run_hydro_command_line(tmp_path, monkeypatch):
