@@ -22,7 +22,7 @@
 #include "gvh_internal.h"
 
 namespace gvh {
-hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, bool mixed,
+hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, int precision,
                         hipStream_t st);
 hipError_t launch_count_above(const double *e, int nx, int ny, int nz, int ncx, int ncy, double thr,
                               unsigned long long *out, hipStream_t st);
@@ -242,7 +242,8 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
   *out = nullptr;
   int rc = check_event_params(lattice, hydro);
   if (rc) return rc;
-  if (opts && opts->precision != GVH_FP64 && opts->precision != GVH_MIXED)
+  if (opts && opts->precision != GVH_FP64 && opts->precision != GVH_MIXED &&
+      opts->precision != GVH_MIXED_F64_ROOTS)
     return gvh_fail(GVH_EINVAL, "unknown precision %d", opts->precision);
 
   gvh_context *c = new gvh_context();
@@ -277,7 +278,7 @@ extern "C" int gvh_create(const gvh_lattice_parameters *lattice, const gvh_hydro
   // one allocation for all state: 3 conserved buffers, 3 x 4 velocity,
   // 2 x 2 (e, p); a mixed conserved buffer holds 4 G doubles + 11 G floats
   const size_t Gs = (size_t)c->G;
-  c->qstride = c->precision == GVH_MIXED ? 4 * Gs + ((11 * Gs + 1) / 2 + 63) / 64 * 64 : 15 * Gs;
+  c->qstride = c->precision != GVH_FP64 ? 4 * Gs + ((11 * Gs + 1) / 2 + 63) / 64 * 64 : 15 * Gs;
   const size_t bytes = (3 * c->qstride + 16 * Gs) * sizeof(double);
   hipError_t e = hipMalloc(&c->arena, bytes);
   if (e != hipSuccess) {
@@ -408,7 +409,7 @@ extern "C" int gvh_set_initial_state(gvh_context *c, double t0, const double *e,
   c->ie = 0;
   const size_t B = sizeof(double);
   std::vector<float> hqf;
-  if (c->precision == GVH_MIXED) {
+  if (c->precision != GVH_FP64) {
     // T^{tau mu} in FP64, pi and Pi stored in FP32
     hqf.assign(hq.begin() + 4 * G, hq.end());
     HIP_TRY(hipMemcpyAsync(c->qbuf[c->iq], hq.data(), 4 * G * B, hipMemcpyHostToDevice, c->stream));
@@ -433,7 +434,7 @@ static void *field_dev(gvh_context *c, int f, bool *is_float) {
   *is_float = false;
   if (f >= GVH_TTT && f <= GVH_PI) {
     double *q = c->qbuf[c->iq];
-    if (c->precision == GVH_MIXED && f >= GVH_PITT) {
+    if (c->precision != GVH_FP64 && f >= GVH_PITT) {
       *is_float = true;
       return reinterpret_cast<float *>(q + 4 * c->G) + (size_t)(f - GVH_PITT) * c->G;
     }
@@ -448,7 +449,7 @@ static void *field_dev(gvh_context *c, int f, bool *is_float) {
 
 int gvh_copy_q_field(gvh_context *c, const double *qbase, int f, double *host) {
   const size_t G = (size_t)c->G;
-  if (c->precision == GVH_MIXED && f >= GVH_PITT) {
+  if (c->precision != GVH_FP64 && f >= GVH_PITT) {
     std::vector<float> tmp(G);
     HIP_TRY(hipMemcpyAsync(tmp.data(), reinterpret_cast<const float *>(qbase + 4 * G) + (f - GVH_PITT) * G,
                            G * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -607,7 +608,7 @@ static int step_impl(gvh_context *c, double t, float *ms1, float *ms2) {
 }
 
 int launch_stage(gvh_context *c, const StageArgs &A, bool stage2) {
-  hipError_t e = launch_stage(A, c->eos.kind, c->bulk, stage2, c->precision == GVH_MIXED, c->stream);
+  hipError_t e = launch_stage(A, c->eos.kind, c->bulk, stage2, c->precision, c->stream);
   if (e != hipSuccess) return gvh_fail(GVH_EHIP, "stage kernel launch: %s", hipGetErrorString(e));
   return GVH_OK;
 }

@@ -521,7 +521,8 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
     const int rc = eos_from_options(opts, H);
     if (rc) return rc;
     if (opts->precision != GVH_FP64 && opts->precision != GVH_MIXED)
-      return gvh_fail(GVH_EINVAL, "unknown precision %d", opts->precision);
+      return gvh_fail(GVH_EINVAL, "a batch runs GVH_FP64 or GVH_MIXED, not precision %d",
+                      opts->precision);
   }
   gvh_batch *b = new gvh_batch();
   b->opts = *opts;

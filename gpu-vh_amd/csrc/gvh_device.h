@@ -596,16 +596,18 @@ __device__ __forceinline__ double flux_velocity(const EosTable &T, double t, dou
 // for the conformal and tabulated EoS: their results only set the flux
 // velocities of FP32 fluxes.  The WB fits (powers up to e^13) overflow FP32
 // and stay FP64, as does every cell's own inversion.
-template <int EOS, typename R>
+// ROOT64 keeps them in FP64 in every form (the mixed form's split as the
+// GVH_MIXED_F64_ROOTS precision states it).
+template <int EOS, typename R, bool ROOT64 = false>
 using flux_root_t =
-    typename std::conditional<(sizeof(R) == 4 && EOS != EOS_WB), float, double>::type;
+    typename std::conditional<(sizeof(R) == 4 && EOS != EOS_WB && !ROOT64), float, double>::type;
 
-template <int EOS, int DIR, int N, typename R>
+template <int EOS, int DIR, int N, typename R, bool ROOT64 = false>
 __device__ __forceinline__ void flux_velocity_lockstep(const EosTable &T, double t,
                                                        const R (*W)[9], double ePrev,
                                                        unsigned long long *diag, int max_iters,
                                                        double *v) {
-  using F = flux_root_t<EOS, R>;
+  using F = flux_root_t<EOS, R, ROOT64>;
   F M0[N], M[N], Md[N], Pi[N], e[N];
   const F tf = F(t);
 #pragma unroll

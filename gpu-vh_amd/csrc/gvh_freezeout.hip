@@ -645,7 +645,7 @@ extern "C" int gvh_freezeout_record(gvh_context *c, int n) {
   const double *u = c->ubuf[stale ? c->iup : c->iu];
   const double *q = c->qbuf[stale ? c->iQ() : c->iq];
   double *dst = f->ring + (size_t)f->slot[logical] * NVAR * f->nint;
-  if (c->precision == GVH_MIXED)
+  if (c->precision != GVH_FP64)
     fo_record_kernel<float><<<grid_for(f->nint, 256), 256, 0, c->stream>>>(
         c->ebuf[c->ie], u, reinterpret_cast<const float *>(q + 4 * c->G), c->G, c->nx, c->ny, c->nz,
         c->ncx, c->ncy, c->bulk ? 1 : 0, dst, f->nint);

@@ -14,15 +14,21 @@ namespace gvh {
 // (FP32 stencil, limiter, flux and sources, FP32 pi/Pi storage, FP32
 // half-site root-finds for the tabulated and conformal EoS; FP64
 // T^{tau mu}, accumulator, the cell's own root-find and its EoS) has
-// arrangements 0, 3 and 4; others run as 0.
+// arrangements 0, 3 and 4; others run as 0.  With root64 (precision
+// GVH_MIXED_F64_ROOTS) the mixed form keeps the half-site root-finds in
+// FP64: arrangement 3 runs its lockstep in FP64, and arrangement 0's
+// one-at-a-time root-finds are FP64 in every form.
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool MIXED>
-static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
+static hipError_t launch_z(const StageArgs &A, bool root64, hipStream_t st) {
   const long long ncell = (long long)A.nx * A.strip * A.nstrips * A.nz;
   const int block = BLOCK;
   const int grid = (int)(((ncell + block - 1) / block + 7) / 8 * 8);  // multiple of 8 (XCD order)
   if constexpr (MIXED) {
     // FP32 stencil, limiter, flux and sources; FP32 pi/Pi storage
-    if (A.variant == 3)
+    if (root64 && A.variant != 0)
+      hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 3, float, float>),
+                         dim3(grid), dim3(block), 0, st, A);
+    else if (A.variant == 3)
       hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 1, float, float>),
                          dim3(grid), dim3(block), 0, st, A);
     else if (A.variant == 4)
@@ -66,33 +72,37 @@ static hipError_t launch_z(const StageArgs &A, hipStream_t st) {
 }
 
 template <int EOS, bool BULK, bool STAGE2, bool MIXED>
-static hipError_t launch_t(const StageArgs &A, bool has_z, hipStream_t st) {
-  return has_z ? launch_z<EOS, BULK, STAGE2, true, MIXED>(A, st)
-               : launch_z<EOS, BULK, STAGE2, false, MIXED>(A, st);
+static hipError_t launch_t(const StageArgs &A, bool has_z, bool root64, hipStream_t st) {
+  return has_z ? launch_z<EOS, BULK, STAGE2, true, MIXED>(A, root64, st)
+               : launch_z<EOS, BULK, STAGE2, false, MIXED>(A, root64, st);
 }
 
 template <int EOS, bool MIXED>
-static hipError_t launch_e(const StageArgs &A, bool bulk, bool stage2, bool has_z, hipStream_t st) {
+static hipError_t launch_e(const StageArgs &A, bool bulk, bool stage2, bool has_z, bool root64,
+                           hipStream_t st) {
   if (bulk)
-    return stage2 ? launch_t<EOS, true, true, MIXED>(A, has_z, st)
-                  : launch_t<EOS, true, false, MIXED>(A, has_z, st);
-  return stage2 ? launch_t<EOS, false, true, MIXED>(A, has_z, st)
-                : launch_t<EOS, false, false, MIXED>(A, has_z, st);
+    return stage2 ? launch_t<EOS, true, true, MIXED>(A, has_z, root64, st)
+                  : launch_t<EOS, true, false, MIXED>(A, has_z, root64, st);
+  return stage2 ? launch_t<EOS, false, true, MIXED>(A, has_z, root64, st)
+                : launch_t<EOS, false, false, MIXED>(A, has_z, root64, st);
 }
 
 template <bool MIXED>
-static hipError_t launch_p(const StageArgs &A, int eos, bool bulk, bool stage2, hipStream_t st) {
+static hipError_t launch_p(const StageArgs &A, int eos, bool bulk, bool stage2, bool root64,
+                           hipStream_t st) {
   const bool has_z = A.nz > 1;
   switch (eos) {
-    case EOS_CONFORMAL: return launch_e<EOS_CONFORMAL, MIXED>(A, bulk, stage2, has_z, st);
-    case EOS_WB: return launch_e<EOS_WB, MIXED>(A, bulk, stage2, has_z, st);
-    default: return launch_e<EOS_TABLE, MIXED>(A, bulk, stage2, has_z, st);
+    case EOS_CONFORMAL: return launch_e<EOS_CONFORMAL, MIXED>(A, bulk, stage2, has_z, root64, st);
+    case EOS_WB: return launch_e<EOS_WB, MIXED>(A, bulk, stage2, has_z, root64, st);
+    default: return launch_e<EOS_TABLE, MIXED>(A, bulk, stage2, has_z, root64, st);
   }
 }
 
-hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, bool mixed,
+// precision: GVH_FP64, GVH_MIXED or GVH_MIXED_F64_ROOTS
+hipError_t launch_stage(const StageArgs &A, int eos, bool bulk, bool stage2, int precision,
                         hipStream_t st) {
-  return mixed ? launch_p<true>(A, eos, bulk, stage2, st) : launch_p<false>(A, eos, bulk, stage2, st);
+  if (precision == GVH_FP64) return launch_p<false>(A, eos, bulk, stage2, false, st);
+  return launch_p<true>(A, eos, bulk, stage2, precision == GVH_MIXED_F64_ROOTS, st);
 }
 
 // Termination scan of the run loop (HydroPlugin.cpp:396-407): count interior

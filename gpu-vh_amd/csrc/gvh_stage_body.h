@@ -339,7 +339,8 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     if constexpr (EARLY_RAW2) load_raw2();
     double v4[4];
     if constexpr (LOCK) {
-      flux_velocity_lockstep<EOS, DIR, 4, R>(A.tab, t, Wst, ePrev, A.diag, A.max_iters, v4);
+      flux_velocity_lockstep<EOS, DIR, 4, R, LOCK == 3>(A.tab, t, Wst, ePrev, A.diag,
+                                                        A.max_iters, v4);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k)

@@ -261,7 +261,7 @@ extern "C" int gvh_check_validity(gvh_context *c, double t, int flags) {
   A.etabar = c->Hp.shearViscosityToEntropyDensity;
   A.zeta_tpeak = c->zeta_tpeak;
   A.tab = c->tab;
-  const bool mixed = c->precision == GVH_MIXED;
+  const bool mixed = c->precision != GVH_FP64;
   hipError_t e;
   switch (c->eos.kind) {
     case EOS_CONFORMAL: e = launch_validity_e<EOS_CONFORMAL>(A, c->bulk, mixed, c->stream); break;

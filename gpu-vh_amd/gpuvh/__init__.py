@@ -21,7 +21,7 @@ EOS_TABLE_PATH = os.path.join(DATA_DIR, "eos_qcdlike_2000.dat")
 HBARC = 0.197326938  # GeV fm (rhic/src/HydroPlugin.cpp:132)
 
 EOS_CONFORMAL, EOS_WB, EOS_TABLE = 0, 1, 2
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_F64_ROOTS = 0, 1, 2
 # status codes (include/gpuvh.h GVH_*)
 OK, EINVAL, EHIP, EIO, ENOMEM, ENOCONV = 0, -1, -2, -3, -4, -5
 

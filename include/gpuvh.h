@@ -86,6 +86,12 @@ int gvh_read_ic_parameters(const char *config_dir, gvh_ic_parameters *out);
                        FP32 half-site root-finds (tabulated and conformal
                        EoS); FP64 T^{tau mu} accumulation, the cell's own
                        root-find and its EoS                                  */
+#define GVH_MIXED_F64_ROOTS 2 /* as GVH_MIXED with every root-find and its
+                                 EoS lookups in FP64 (the 12 half-site ones
+                                 included): the split of FP32 flux, limiter,
+                                 source and stencil arithmetic and FP32 pi, Pi
+                                 storage against FP64 accumulation, root-find
+                                 and EoS inversion.  Single contexts only.   */
 
 typedef struct gvh_options {
   int eos;             /* GVH_EOS_*                                         */

@@ -1,19 +1,24 @@
-"""Mixed-precision form (FP32 stencil, limiter, flux, half-site root-finds,
-sources and pi/Pi storage; FP64 T^{tau mu} accumulation and the cell's own
-root-find) against the FP64 form, which matches the oracle to 1e-10.
+"""Mixed-precision forms against the oracle (the FP64 CPU restatement of the
+reference KT step):
+
+- GVH_MIXED: FP32 stencil, limiter, flux, sources, pi/Pi storage and the 12
+  half-site root-finds (tabulated and conformal EoS); FP64 T^{tau mu}
+  accumulation and the cell's own root-find;
+- GVH_MIXED_F64_ROOTS: the same with every root-find and its EoS lookups in
+  FP64 (the split of FP32 flux, limiter, source and stencil arithmetic and
+  FP32 pi, Pi storage against FP64 accumulation and root-finds).
 
 Metric: per-cell relative error with a floor of MIXED_FLOOR (1%) of the
 field's scale -- components that cross zero (u^x, u^y, u^eta, pi) are
 resolved to FP32 precision in absolute terms, so below 1% of their maximum
 the bar becomes absolute (1e-6 of the maximum).  Bound: 1e-4 for e and u^mu
-(the tolerance stated for the mixed form), 1e-3 for pi^{mu nu}.  The Lorentz-norm error
-max |du^mu| / u^tau is reported beside it."""
+(the tolerance stated for the mixed form), 1e-3 for pi^{mu nu}.  The
+Lorentz-norm error max |du^mu| / u^tau is reported beside it."""
 import numpy as np
 import pytest
 
 import gpuvh
-from parity import (FLOOR_FRAC, Case, PI_FIELDS, U_FIELDS, compare, floor_scales, hotspot_state,
-                    rel_err)
+from parity import Case, PI_FIELDS, U_FIELDS, floor_scales, hotspot_state, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -44,21 +49,35 @@ CASES = [
 ]
 
 
+_ORACLE = {}
+
+
+def oracle_fields(name, case, state, nsteps, fields):
+    """The oracle's fields after nsteps, computed once per case."""
+    if name not in _ORACLE:
+        orc = case.oracle()
+        orc.set_primitives(case.t0, *state)
+        orc.steps(case.t0, 0, nsteps)
+        _ORACLE[name] = {f: orc.get(f) for f in fields}
+        orc.close()
+    return _ORACLE[name]
+
+
+@pytest.mark.parametrize("precision", [gpuvh.MIXED, gpuvh.MIXED_F64_ROOTS],
+                         ids=["mixed", "mixed_f64_roots"])
 @pytest.mark.parametrize("variant", [0, 3])
 @pytest.mark.parametrize("name,case,nsteps", CASES, ids=[c[0] for c in CASES])
-def test_mixed_against_fp64(name, case, nsteps, variant, monkeypatch):
-    """After nsteps the mixed form's e and u^mu agree with the FP64 form (itself
-    the oracle's to 1e-10, and checked against it here) within 1e-4 per cell;
-    pi^{mu nu} within 1e-3 of its scale."""
-    say("mixed", name, "variant", variant, "steps", nsteps)
+def test_mixed_against_oracle(name, case, nsteps, variant, precision, monkeypatch):
+    """After nsteps the mixed forms' e and u^mu agree with the oracle within
+    1e-4 per cell; pi^{mu nu} within 1e-3 of its scale."""
+    say("mixed", name, "variant", variant, "precision", precision, "steps", nsteps)
     state = hotspot_state(case, seed=5, n_hot=10, amp=6.0, u_noise=0.02)
+    fields = [gpuvh.E] + U_FIELDS + PI_FIELDS
+    refs = oracle_fields(name, case, state, nsteps, fields)
     monkeypatch.setenv("GVH_VARIANT", str(variant))
-    ref = run_ctx(case, state, nsteps, gpuvh.FP64)
-    mix = run_ctx(case, state, nsteps, gpuvh.MIXED)
+    mix = run_ctx(case, state, nsteps, precision)
     tau = case.t0 + nsteps * case.dt
     errs = {}
-    fields = [gpuvh.E] + U_FIELDS + PI_FIELDS
-    refs = {f: ref.get(f) for f in fields}
     scales = floor_scales(refs, tau)
     for f in fields:
         errs[gpuvh.FIELD_NAMES[f]] = rel_err(mix.get(f), refs[f], floor_frac=MIXED_FLOOR,
@@ -76,7 +95,6 @@ def test_mixed_against_fp64(name, case, nsteps, variant, monkeypatch):
     assert np.array_equal(pixx, pixx.astype(np.float32).astype(np.float64))
     d = mix.diagnostics()
     assert d["e_nan"] == 0 and d["newton_not_converged"] == 0
-    ref.close()
     mix.close()
 
 
