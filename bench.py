@@ -69,28 +69,14 @@ def init_state(ctx, args, e):
     ctx.set_initial_state(args.t0, e, p, ut, ux, uy, un, pi10, Pi)
 
 
-def time_steps(ctx, args):
-    """Warm-up, then args.steps timed steps (wall clock, synchronized on both
-    sides), then args.timed_steps steps with HIP events around each stage
-    kernel.  Returns (seconds, [stage-1 ms], [stage-2 ms])."""
-    import torch
+def newton_fraction(ctx):
+    """Fraction of interior cells whose root-finds take the Newton branch:
+    the inversions of a cell use its own e as the guess, and a guess
+    <= 0.1 fm^-4 selects the closed form e = M0 - M/M0
+    (EnergyMomentumTensor.cu:87-88)."""
+    import gpuvh
 
-    n = 0
-    ctx.steps(args.t0, n, args.warmup)
-    n += args.warmup
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    ctx.steps(args.t0, n, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    n += args.steps
-    s1, s2 = [], []
-    for _ in range(args.timed_steps):
-        a, b = ctx.step_timed(args.t0 + n * args.dt)
-        n += 1
-        s1.append(a)
-        s2.append(b)
-    return elapsed, s1, s2
+    return float(np.mean(ctx.get(gpuvh.E) > 0.1))
 
 
 VALIDITY_BYTES_PER_CELL = 31 * 8  # reads e, p, u, u_prev, 10 pi, Pi; writes 10 values
@@ -101,7 +87,7 @@ def time_validity(ctx, args, stream, ncells, reps=10):
     events on the context's stream around reps launches."""
     import torch
 
-    t = args.t0 + (args.warmup + args.steps + args.timed_steps) * args.dt
+    t = args.t0 + (args.warmup + args.steps) * args.dt
     ctx.validity_enable()
     ctx.check_validity(t)
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -167,6 +153,23 @@ def mixed_error(args, e, stream, nsteps):
         out[name] = worst
     ut = ctxs[gpuvh.FP64].get(gpuvh.UT)
     out["u_lorentz"] = du / float(np.nanmin(np.abs(ut)))
+    # the north-star criterion: the fluid at or above the freeze-out energy
+    # density (T >= 0.15 GeV)
+    e_fo = ctxs[gpuvh.FP64].energy_density_of_T(0.15 / gpuvh.HBARC)
+    ref = {f: ctxs[gpuvh.FP64].get(f) for f in groups["u"] + groups["pi"] + [gpuvh.E]}
+    got = {f: ctxs[gpuvh.MIXED].get(f) for f in ref}
+    m = ref[gpuvh.E] >= e_fo
+    fo = {"cells": int(m.sum())}
+    if m.any():
+        fo["e"] = float("%.3e" % np.max(np.abs(got[gpuvh.E][m] - ref[gpuvh.E][m]) / ref[gpuvh.E][m]))
+        tau = args.t0 + nsteps * args.dt
+        fo["u"] = float("%.3e" % max(
+            float(np.max(np.abs(got[f][m] - ref[f][m]) * (tau if f == gpuvh.UN else 1.0)
+                         / ref[gpuvh.UT][m])) for f in groups["u"]))
+        pscale = max(float(np.max(np.abs(ref[f][m]))) for f in groups["pi"])
+        fo["pi"] = float("%.3e" % (max(float(np.max(np.abs(got[f][m] - ref[f][m])))
+                                       for f in groups["pi"]) / pscale))
+    out["at_or_above_e_fo"] = fo
     out["nonfinite_cells"] = nonfinite
     out["diagnostics_fp64"] = ctxs[gpuvh.FP64].diagnostics()
     out["diagnostics_mixed"] = ctxs[gpuvh.MIXED].diagnostics()
@@ -202,14 +205,15 @@ def batch_workload(n_events, seed, kind):
     return out
 
 
-def run_batch_mode(args):
-    """64 2+1D events in one batch (mixed precision, autotuned) against the
-    same events one by one in plain FP64: throughput and error."""
+def run_batch_mode(args, n_events):
+    """2+1D events in one batch (mixed precision, autotuned) against the
+    same events one by one in plain FP64: throughput and error.  Returns the
+    summary line as a dict."""
     import torch
     import gpuvh
 
     table = gpuvh.EosTable()
-    events = batch_workload(args.events, args.seed, "batch")
+    events = batch_workload(n_events, args.seed, "batch")
     cells = sum(L.numLatticePointsX * L.numLatticePointsY * (L.numProperTimePoints + 1)
                 for L, H, e in events)
     log("batch: %d events, %.3g cell-updates" % (len(events), cells))
@@ -268,16 +272,17 @@ def run_batch_mode(args):
            "mixed_error_vs_fp64_alone": {k: float("%.3e" % v) for k, v in errs.items()},
            "error_metric": "max per-cell |mixed - fp64| / max(|fp64|, 1e-2 max|fp64|) over the "
                            "first %d events at their last step" % keep}
-    print(json.dumps(out))
+    return out
 
 
-def run_serve_mode(args):
+def run_serve_mode(args, n_events):
     """Serving loop: a Poisson stream of events at 80% of the measured
-    capacity into a continuously batched engine; events/s and latency."""
+    capacity into a continuously batched engine; events/s and latency.
+    Returns the summary line as a dict."""
     import gpuvh
 
     table = gpuvh.EosTable()
-    events = batch_workload(args.events, args.seed, "serve")
+    events = batch_workload(n_events, args.seed, "serve")
     report = {}
     for prec, name in ((gpuvh.MIXED, "mixed"), (gpuvh.FP64, "fp64")):
         opts = gpuvh.options(gpuvh.EOS_TABLE, True, prec, 0.16 / gpuvh.HBARC, table=table)
@@ -329,15 +334,42 @@ def run_serve_mode(args):
                                   "freeze-out at T_fo 0.15 GeV with surfaces, Poisson arrivals"
                                   % len(events), "seed": args.seed},
            "mixed": report["mixed"], "fp64": report["fp64"]}
-    print(json.dumps(out))
+    return out
+
+
+def cpu_info():
+    """CPU model and core counts of this host from /proc/cpuinfo."""
+    model, phys, logical = None, set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = core = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "processor":
+                    logical += 1
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    core = v
+                    phys.add((pid, core))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": logical or os.cpu_count(),
+            "physical_cores": len(phys) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 def cpu_baseline(args, e_full, threads):
     """Time the oracle (FP64 CPU restatement of the reference KT step, test
-    infrastructure) on the host cores: KT steps of the same event on the full
-    lattice.  Cell-updates/s."""
+    infrastructure) on the host cores, threads pinned close: KT steps of the
+    same event on the full lattice, after one untimed step.  Cell-updates/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ.setdefault("OMP_PLACES", "cores")
     import gpuvh
     import oracle_lib as O
     from gpuvh import ic
@@ -349,13 +381,57 @@ def cpu_baseline(args, e_full, threads):
     ut, ux, uy, un, pi10, Pi = ic.at_rest(e_full.shape)
     p = O.eos_p_batch(P, e_full)
     orc.set_primitives(args.t0, e_full, p, ut, ux, uy, un, pi10, Pi)
+    orc.steps(args.t0, 0, 1)
     nsteps = args.cpu_steps
     t0 = time.perf_counter()
-    orc.steps(args.t0, 0, nsteps)
+    orc.steps(args.t0, 1, nsteps)
     el = time.perf_counter() - t0
     orc.close()
     cells = args.nx * args.ny * args.nz
     return cells * nsteps / el, el, nsteps
+
+
+def dense_event(args):
+    """The benchmark lattice with no cell near vacuum (ic.c2_dense_event):
+    the 200 hotspots on a 15 GeV/fm^3 transverse Gaussian base (sigma 6 fm)
+    and an eta plateau wider than the lattice; it runs 500 steps."""
+    from gpuvh import ic
+
+    return ic.c2_dense_event(args.nx, args.ny, args.nz, 0.1, 0.1, 0.1, seed=args.seed)
+
+
+def run_form(args, form, eos, stream, e, ncells, want_validity=False):
+    """One form on one event: timed steps, kernel times from the same steps,
+    the Newton-branch fraction at the start and end of the timed steps."""
+    import gpuvh
+
+    prec = {"fp64": gpuvh.FP64, "mixed": gpuvh.MIXED,
+            "mixed_f64_roots": gpuvh.MIXED_F64_ROOTS}[form]
+    ctx = make_context(args, prec, eos, stream.cuda_stream)
+    init_state(ctx, args, e)
+    ctx.steps(args.t0, 0, args.warmup)
+    nf0 = newton_fraction(ctx)
+    import torch
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    s1, s2, gpu_ms = ctx.steps_timed(args.t0, args.warmup, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    s1, s2 = s1 / args.steps, s2 / args.steps
+    nf1 = newton_fraction(ctx)
+    diag = ctx.diagnostics()
+    validity = time_validity(ctx, args, stream, ncells) if want_validity else None
+    ctx.close()
+    bpc = algorithmic_bytes_per_cell_update(True, "fp64" if form == "fp64" else "mixed")
+    kt_ms = s1 + s2
+    log("%s: %.4f ms/step, stage kernels %.4f + %.4f ms, Newton branch %.3f -> %.3f"
+        % (form, 1e3 * elapsed / args.steps, s1, s2, nf0, nf1))
+    return {"ms_per_step": 1e3 * elapsed / args.steps, "value": ncells * args.steps / elapsed,
+            "stage_ms": [s1, s2], "kernel_ms": kt_ms, "gpu_ms_per_step": gpu_ms / args.steps,
+            "bytes_per_cell_update": bpc,
+            "achieved_gbs": bpc * ncells / (kt_ms * 1e-3) / 1e9,
+            "newton_fraction": [round(nf0, 4), round(nf1, 4)], "diagnostics": diag,
+            "validity": validity}
 
 
 def main():
@@ -369,25 +445,29 @@ def main():
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--t0", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_f64_roots"], default="fp64",
                     help="form of the headline line (the other is reported beside it)")
     ap.add_argument("--eos", choices=["table", "wb", "conformal"], default="table",
                     help="equation of state (the workload's is the 2000-node table)")
     ap.add_argument("--variant", type=int, default=None,
                     help="stage-kernel arrangement (library default if unset)")
-    ap.add_argument("--timed-steps", type=int, default=5,
-                    help="extra steps with HIP events around each stage kernel")
     ap.add_argument("--error-steps", type=int, default=50,
                     help="steps after which mixed is compared with FP64 (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision report")
-    ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense-event line")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (default: OMP_NUM_THREADS, else the CPUs of this process)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", action="store_true",
-                    help="batched 2+1D events against one-by-one FP64 (prints its own line)")
+                    help="only the batched 2+1D events against one-by-one FP64 (own line)")
     ap.add_argument("--serve", action="store_true",
-                    help="serving loop with Poisson arrivals (prints its own line)")
+                    help="only the serving loop with Poisson arrivals (own line)")
     ap.add_argument("--events", type=int, default=64, help="events of --batch / --serve")
+    ap.add_argument("--batch-events", type=int, default=64,
+                    help="events of the batch summary in the default line (0: skip)")
+    ap.add_argument("--serve-events", type=int, default=24,
+                    help="events of the serving summary in the default line (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -407,10 +487,10 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     if args.batch:
-        run_batch_mode(args)
+        print(json.dumps(run_batch_mode(args, args.events)))
         return
     if args.serve:
-        run_serve_mode(args)
+        print(json.dumps(run_serve_mode(args, args.events)))
         return
 
     if args.copy_calibration:
@@ -425,29 +505,10 @@ def main():
     eos = {"table": gpuvh.EOS_TABLE, "wb": gpuvh.EOS_WB, "conformal": gpuvh.EOS_CONFORMAL}[args.eos]
     ncells = args.nx * args.ny * args.nz
     forms = [args.precision] + ([] if args.no_mixed else
-                                [p for p in ("fp64", "mixed") if p != args.precision])
-    res = {}
-    diag = None
-    validity = None
-    for form in forms:
-        prec = gpuvh.FP64 if form == "fp64" else gpuvh.MIXED
-        ctx = make_context(args, prec, eos, stream.cuda_stream)
-        init_state(ctx, args, e)
-        elapsed, s1, s2 = time_steps(ctx, args)
-        if form == args.precision:
-            diag = ctx.diagnostics()
-        if form == "fp64":
-            validity = time_validity(ctx, args, stream, ncells)
-        ctx.close()
-        bpc = algorithmic_bytes_per_cell_update(True, form)
-        kt_ms = float(np.mean(s1) + np.mean(s2))
-        log("%s: %.4f ms/step, stage kernels %.4f + %.4f ms"
-            % (form, 1e3 * elapsed / args.steps, np.mean(s1), np.mean(s2)))
-        res[form] = {"ms_per_step": 1e3 * elapsed / args.steps,
-                     "value": ncells * args.steps / elapsed,
-                     "stage_ms": [float(np.mean(s1)), float(np.mean(s2))],
-                     "kernel_ms": kt_ms, "bytes_per_cell_update": bpc,
-                     "achieved_gbs": bpc * ncells / (kt_ms * 1e-3) / 1e9}
+                                [p for p in ("fp64", "mixed", "mixed_f64_roots")
+                                 if p != args.precision])
+    res = {form: run_form(args, form, eos, stream, e, ncells, want_validity=(form == "fp64"))
+           for form in forms}
 
     head = res[args.precision]
     wkey = "%dx%dx%d/%s/%s/v%s" % (args.nx, args.ny, args.nz, args.eos, args.precision,
@@ -462,6 +523,7 @@ def main():
                 "traffic_source": rec["source"] if rec else None,
                 "kernel": "kt_flux_kernel (one launch per RK stage)",
                 "stage_ms": [round(v, 4) for v in head["stage_ms"]],
+                "timing": "HIP events around every stage kernel of the %d timed steps" % args.steps,
                 "bytes_per_cell_update": head["bytes_per_cell_update"],
                 "cells_per_launch": ncells}
 
@@ -472,7 +534,18 @@ def main():
         mixed = {"ms_per_step": round(m["ms_per_step"], 4), "value": round(m["value"], 1),
                  "speedup_vs_fp64": round(res["fp64"]["ms_per_step"] / m["ms_per_step"], 3),
                  "roofline_frac": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                 "bytes_per_cell_update": m["bytes_per_cell_update"], "reported_beside": other}
+                 "stage_ms": [round(v, 4) for v in m["stage_ms"]],
+                 "bytes_per_cell_update": m["bytes_per_cell_update"], "reported_beside": other,
+                 "split": "FP32 stencil, limiter, flux, sources, pi/Pi storage and the 12 "
+                          "half-site root-finds; FP64 T^{tau mu} accumulation, the cell's own "
+                          "root-find and its EoS"}
+        if "mixed_f64_roots" in res:
+            m2 = res["mixed_f64_roots"]
+            mixed["f64_roots"] = {
+                "split": "as mixed, with every root-find and its EoS lookups in FP64",
+                "ms_per_step": round(m2["ms_per_step"], 4),
+                "speedup_vs_fp64": round(res["fp64"]["ms_per_step"] / m2["ms_per_step"], 3),
+                "stage_ms": [round(v, 4) for v in m2["stage_ms"]]}
         if args.error_steps > 0 and args.eos == "table":
             log("mixed vs fp64 after %d steps" % args.error_steps)
             errs = mixed_error(args, e, stream.cuda_stream, args.error_steps)
@@ -480,19 +553,51 @@ def main():
                                       for k, v in errs.items()}
             mixed["error_steps"] = args.error_steps
             mixed["error_metric"] = ("max per-cell |mixed - fp64| / max(|fp64|, 1e-2 max|fp64|) "
-                                     "per field group; u_lorentz = max|du^i| / min u^tau")
+                                     "per field group; u_lorentz = max|du^i| / min u^tau; "
+                                     "at_or_above_e_fo: on cells with e >= e(T = 0.15 GeV), "
+                                     "e relative, u max|du^mu|/u^tau, pi relative to max|pi|")
+
+    dense = None
+    if not args.no_dense and args.eos == "table":
+        ed = dense_event(args)
+        log("dense event built")
+        dres = {form: run_form(args, form, eos, stream, ed, ncells)
+                for form in (["fp64"] + ([] if args.no_mixed else ["mixed"]))}
+        d = dres["fp64"]
+        dense = {"workload": "the same lattice, 200 hotspots on a 15 GeV/fm^3 Gaussian base "
+                             "(sigma 6 fm), eta plateau wider than the lattice (runs 500 steps)",
+                 "value": round(d["value"], 1), "ms_per_step": round(d["ms_per_step"], 4),
+                 "stage_ms": [round(v, 4) for v in d["stage_ms"]],
+                 "roofline_frac": round(d["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                 "newton_fraction": d["newton_fraction"], "diagnostics": d["diagnostics"]}
+        if "mixed" in dres:
+            dense["mixed_ms_per_step"] = round(dres["mixed"]["ms_per_step"], 4)
+            dense["mixed_speedup_vs_fp64"] = round(d["ms_per_step"] / dres["mixed"]["ms_per_step"], 3)
+
+    batch = serve = None
+    if args.batch_events > 0 and args.eos == "table":
+        log("batch summary: %d events" % args.batch_events)
+        batch = run_batch_mode(args, args.batch_events)
+    if args.serve_events > 0 and args.eos == "table":
+        log("serve summary: %d events" % args.serve_events)
+        serve = run_serve_mode(args, args.serve_events)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline: %d oracle steps on %d threads" % (args.cpu_steps, threads))
+        env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = args.cpu_threads or env_threads or len(os.sched_getaffinity(0))
+        info = cpu_info()
+        log("cpu baseline: %d oracle steps on %d threads (%s)" % (args.cpu_steps, threads, info))
         v, el, ns = cpu_baseline(args, e, threads)
         log("cpu baseline: %.3g cell-updates/s" % v)
         cpu = {"value": round(v, 1), "unit": "cell-updates/s", "cores": threads,
                "kind": "port",
-               "sample": "oracle (FP64 CPU restatement of the reference KT step, OpenMP) on "
-                         "the full %dx%dx%d event, %d KT step(s), %.1f s"
-                         % (args.nx, args.ny, args.nz, ns, el)}
+               "sample": "oracle (FP64 CPU restatement of the reference KT step, OpenMP, "
+                         "OMP_PROC_BIND=close) on the full %dx%dx%d event, %d KT steps after one "
+                         "untimed step, %.1f s" % (args.nx, args.ny, args.nz, ns, el),
+               "cpu": info,
+               "threads_note": "threads = OMP_NUM_THREADS of the environment (the host's CPU "
+                               "share for one GPU) unless --cpu-threads is given"}
 
     out = {
         "metric": "cell-updates/s per KT step",
@@ -516,9 +621,13 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "newton_fraction": head["newton_fraction"],
         "mixed": mixed,
-        "validity": validity,
-        "diagnostics": diag,
+        "dense": dense,
+        "batch": batch,
+        "serve": serve,
+        "validity": res["fp64"]["validity"] if "fp64" in res else None,
+        "diagnostics": head["diagnostics"],
     }
     print(json.dumps(out))
 

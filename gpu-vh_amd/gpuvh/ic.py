@@ -105,6 +105,23 @@ def c2_event(nx=201, ny=201, nz=101, dx=0.1, dy=0.1, dz=0.1, seed=2,
     return quantise_like_reference(eT[:, :, None] * eL[None, None, :])
 
 
+def c2_dense_event(nx=201, ny=201, nz=101, dx=0.1, dy=0.1, dz=0.1, seed=2, eta_flat=10.2,
+                   base_peak=15.0, base_sigma=6.0):
+    """The 3+1D event's 200 hotspots (as c2_event) on a transverse Gaussian
+    base (base_peak GeV/fm^3, sigma base_sigma fm) times the reference's
+    eta profile with a plateau of width eta_flat, by default wider than the
+    lattice, so that no cell starts near vacuum.  c2_event's eta tails fall
+    to the reader's 1e-3 fm^-4 floor and the viscous evolution goes unstable
+    there near step 90 (the reference scheme's own behaviour); on the
+    201x201x101 lattice this form runs 500 steps (a plateau of width 8, with
+    tails on the lattice, fails near step 125).  Shape (nx, ny, nz), fm^-4,
+    quantised like the reference's file reader."""
+    eT = hotspot_event_2d(nx, ny, dx, dy, seed, n_hot=200, hot_sigma=0.5, hot_amp_max=5.0,
+                          base_peak=base_peak, base_sigma=base_sigma, centre_radius=4.0)
+    eL = longitudinal_profile(nz, dz, eta_flat=eta_flat)
+    return quantise_like_reference(eT[:, :, None] * eL[None, None, :])
+
+
 def serving_event(nx, ny, nz, dx, dz, seed, n_hot, base_peak=15.0, hot_amp_max=5.0):
     """Event of the batched and serving workloads: a transverse Gaussian base
     (base_peak GeV/fm^3, sigma 0.45 of the lattice half-width, so no cell
