@@ -36,9 +36,14 @@
 #ifndef GVH_WAVES_FP64
 #define GVH_WAVES_FP64 1
 #endif
-template <typename R>
+// The mixed form with FP64 half-site root-finds (arrangement LOCK 3): its
+// FP64 lockstep chains spill 212 B per lane at three waves.
+#ifndef GVH_WAVES_FP32_ROOT64
+#define GVH_WAVES_FP32_ROOT64 2
+#endif
+template <typename R, int LOCK = 0>
 constexpr int stage_waves_per_simd() {
-  return sizeof(R) == 4 ? GVH_WAVES_FP32 : GVH_WAVES_FP64;
+  return sizeof(R) == 4 ? (LOCK == 3 ? GVH_WAVES_FP32_ROOT64 : GVH_WAVES_FP32) : GVH_WAVES_FP64;
 }
 
 namespace gvh {
@@ -703,7 +708,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
 
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, int LOCK,
           typename R, typename PS>
-__global__ __launch_bounds__(BLOCK, stage_waves_per_simd<R>()) void kt_flux_kernel(StageArgs A) {
+__global__ __launch_bounds__(BLOCK, (stage_waves_per_simd<R, LOCK>())) void kt_flux_kernel(StageArgs A) {
   __shared__ double acc_lds[15 * BLOCK];
   const CellPos c = cell_of_thread(A);
   if (!c.valid) return;
