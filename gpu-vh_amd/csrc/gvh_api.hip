@@ -500,21 +500,28 @@ extern "C" int gvh_get_field(gvh_context *c, int field, int ghosted, double *hos
     }
     return GVH_OK;
   }
-  const double *d = (const double *)dv;
+  return gvh_copy_ghosted_field(c, (const double *)dv, ghosted, host_out);
+}
+
+// One ghosted block of G doubles to the host: the whole block with one copy,
+// then (ghosted == 0) the interior cells picked on the host.  Strided 2-D
+// copies into pageable memory returned stale rows when a profiler was
+// attached, so none are used.
+int gvh_copy_ghosted_field(gvh_context *c, const double *d, int ghosted, double *host_out) {
   if (ghosted) {
     HIP_TRY(hipMemcpyAsync(host_out, d, (size_t)c->G * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return GVH_OK;
   }
-  // interior rows: one 2-D copy per eta slice
-  for (int k = 0; k < c->nz; ++k) {
-    const double *src = d + 2 + (size_t)c->ncx * (2 + (size_t)c->ncy * (k + 2));
-    double *dst = host_out + (size_t)c->nx * c->ny * k;
-    HIP_TRY(hipMemcpy2DAsync(dst, c->nx * sizeof(double), src, c->ncx * sizeof(double),
-                             c->nx * sizeof(double), c->ny, hipMemcpyDeviceToHost, c->stream));
-  }
+  std::vector<double> full(c->G);
+  HIP_TRY(hipMemcpyAsync(full.data(), d, (size_t)c->G * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nz; ++k)
+    for (int j = 0; j < c->ny; ++j)
+      std::copy_n(full.data() + 2 + (size_t)c->ncx * ((j + 2) + (size_t)c->ncy * (k + 2)), c->nx,
+                  host_out + (size_t)c->nx * (j + (size_t)c->ny * k));
   return GVH_OK;
 }
 

@@ -36,6 +36,9 @@ struct gvh_context;
 // Conserved field f (GVH_TTT..GVH_PI) of conserved buffer qbase as doubles on
 // the full ghosted grid, whatever its storage precision.
 int gvh_copy_q_field(gvh_context *c, const double *qbase, int f, double *host_ghosted);
+// A ghosted block of G doubles on the device to the host: all of it
+// (ghosted != 0) or the interior cells, x fastest.
+int gvh_copy_ghosted_field(gvh_context *c, const double *d, int ghosted, double *host_out);
 
 namespace gvh {
 // Stage launch arguments of a context for stage 1 or 2 of the step at t.

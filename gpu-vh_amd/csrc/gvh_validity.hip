@@ -278,18 +278,5 @@ extern "C" int gvh_get_validity(gvh_context *c, int field, int ghosted, double *
     return gvh_fail(GVH_EINVAL, "unknown validity field %d", field);
   if (!c->validity)
     return gvh_fail(GVH_EINVAL, "validity arrays not enabled (gvh_check_validity)");
-  const double *d = c->validity + (size_t)field * c->G;
-  if (ghosted) {
-    HIP_TRY(hipMemcpyAsync(host_out, d, (size_t)c->G * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-  } else {
-    for (int k = 0; k < c->nz; ++k) {
-      const double *src = d + 2 + (size_t)c->ncx * (2 + (size_t)c->ncy * (k + 2));
-      HIP_TRY(hipMemcpy2DAsync(host_out + (size_t)c->nx * c->ny * k, c->nx * sizeof(double), src,
-                               c->ncx * sizeof(double), c->nx * sizeof(double), c->ny,
-                               hipMemcpyDeviceToHost, c->stream));
-    }
-  }
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return GVH_OK;
+  return gvh_copy_ghosted_field(c, c->validity + (size_t)field * c->G, ghosted, host_out);
 }
