@@ -154,22 +154,32 @@ def mixed_error(args, e, stream, nsteps):
     ut = ctxs[gpuvh.FP64].get(gpuvh.UT)
     out["u_lorentz"] = du / float(np.nanmin(np.abs(ut)))
     # the north-star criterion: the fluid at or above the freeze-out energy
-    # density (T >= 0.15 GeV)
+    # density (T >= 0.15 GeV); and where the global errors come from, by the
+    # FP64 energy density of the cell
     e_fo = ctxs[gpuvh.FP64].energy_density_of_T(0.15 / gpuvh.HBARC)
     ref = {f: ctxs[gpuvh.FP64].get(f) for f in groups["u"] + groups["pi"] + [gpuvh.E]}
     got = {f: ctxs[gpuvh.MIXED].get(f) for f in ref}
-    m = ref[gpuvh.E] >= e_fo
-    fo = {"cells": int(m.sum())}
-    if m.any():
-        fo["e"] = float("%.3e" % np.max(np.abs(got[gpuvh.E][m] - ref[gpuvh.E][m]) / ref[gpuvh.E][m]))
-        tau = args.t0 + nsteps * args.dt
-        fo["u"] = float("%.3e" % max(
+    tau = args.t0 + nsteps * args.dt
+
+    def region(m):
+        r = {"cells": int(m.sum())}
+        if not m.any():
+            return r
+        r["e"] = float("%.3e" % np.max(np.abs(got[gpuvh.E][m] - ref[gpuvh.E][m]) / ref[gpuvh.E][m]))
+        r["u"] = float("%.3e" % max(
             float(np.max(np.abs(got[f][m] - ref[f][m]) * (tau if f == gpuvh.UN else 1.0)
                          / ref[gpuvh.UT][m])) for f in groups["u"]))
         pscale = max(float(np.max(np.abs(ref[f][m]))) for f in groups["pi"])
-        fo["pi"] = float("%.3e" % (max(float(np.max(np.abs(got[f][m] - ref[f][m])))
-                                       for f in groups["pi"]) / pscale))
-    out["at_or_above_e_fo"] = fo
+        if pscale > 0:
+            r["pi"] = float("%.3e" % (max(float(np.max(np.abs(got[f][m] - ref[f][m])))
+                                          for f in groups["pi"]) / pscale))
+        return r
+
+    eref = ref[gpuvh.E]
+    out["at_or_above_e_fo"] = region(eref >= e_fo)
+    out["by_energy_density"] = {"e >= e_fo": out["at_or_above_e_fo"],
+                                "0.1 <= e < e_fo": region((eref >= 0.1) & (eref < e_fo)),
+                                "e < 0.1 (vacuum branch)": region(eref < 0.1)}
     out["nonfinite_cells"] = nonfinite
     out["diagnostics_fp64"] = ctxs[gpuvh.FP64].diagnostics()
     out["diagnostics_mixed"] = ctxs[gpuvh.MIXED].diagnostics()
