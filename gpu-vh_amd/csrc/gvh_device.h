@@ -169,12 +169,38 @@ __device__ __forceinline__ void st_cell(T *base, unsigned s, T v) {
 // = 2 min(|a|, |c|, |b|/4) exactly, and a, c in one class force b into it
 // (y >= x and z >= y give z >= x, likewise for <), so the class test needs
 // only a and c.
+//
+// Every use takes half of it (the half-site states q +- D/2) or twice the
+// half (the gradients), so the kernels evaluate D/2 = sign min(|a|, |c|,
+// |b|/4) directly; the factors 2 and 1/2 cancel exactly.
+// FP32: the two-argument minmod is the median of (p, q, 0), so
+// D/2 = med3(a, med3(b/4, c, 0), 0), two v_med3_f32.
+// FP64 (there is no v_med3_f64): the magnitude from two v_min_f64, the sign
+// of a inserted into its high word, zero when a and c differ in sign bit.
+// A zero a or c makes the magnitude zero, so classing -0 with the negatives
+// changes at most the sign of a zero result.
+static_assert(THETA == 2.0, "the half-slope form assumes theta = 2");
+template <typename R>
+__device__ __forceinline__ R half_slope_t(R x, R y, R z);
+template <>
+__device__ __forceinline__ float half_slope_t<float>(float x, float y, float z) {
+  const float a = y - x, b = z - x, c = z - y;
+  return __builtin_amdgcn_fmed3f(a, __builtin_amdgcn_fmed3f(b * 0.25f, c, 0.0f), 0.0f);
+}
+template <>
+__device__ __forceinline__ double half_slope_t<double>(double x, double y, double z) {
+  const double a = y - x, b = z - x, c = z - y;
+  const double m = fmin(fmin(fabs(a), fabs(c)), fabs(b) * 0.25);
+  int sx = __double2hiint(a) ^ __double2hiint(c);
+  // an empty register constraint keeps the sign test one 32-bit xor and
+  // compare; left visible, the combiner widens it to a 64-bit xor and compare
+  asm("" : "+v"(sx));
+  return sx >= 0 ? copysign(m, a) : 0.0;
+}
 template <typename R>
 __device__ __forceinline__ R slope_t(R x, R y, R z) {
-  const R a = y - x, b = z - x, c = z - y;
-  const R m = R(THETA) * fmin(fmin(fabs(a), fabs(c)), fabs(b) * R(0.25));
-  const bool na = a < R(0), nc = c < R(0);
-  return na == nc ? (na ? -m : m) : R(0);
+  const R h = half_slope_t<R>(x, y, z);
+  return h + h;
 }
 __device__ __forceinline__ double slope(double x, double y, double z) {
   return slope_t<double>(x, y, z);

@@ -334,11 +334,11 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
       }
       const R vm2 = raw[a][0], vm1 = raw[a][1], v0 = raw[a][2], vp1 = raw[a][3],
               vp2 = raw[a][4];
-      const R D0 = slope_t<R>(vm1, v0, vp1);
-      Wst[0][a] = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
-      Wst[1][a] = v0 + D0 * R(0.5);
-      Wst[2][a] = v0 - D0 * R(0.5);
-      Wst[3][a] = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
+      const R h0 = half_slope_t<R>(vm1, v0, vp1);
+      Wst[0][a] = vp1 - half_slope_t<R>(v0, vp1, vp2);
+      Wst[1][a] = v0 + h0;
+      Wst[2][a] = v0 - h0;
+      Wst[3][a] = vm1 + half_slope_t<R>(vm2, vm1, v0);
       if (a >= 4) cdv[a - 4] = (vp1 - vm1) * hc;
     }
     if constexpr (EARLY_RAW2) load_raw2();
@@ -364,7 +364,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
 #pragma unroll 1
     for (int state = 0; state < 4; ++state) {
       const long long o = state == 0 ? st : (state == 3 ? -st : 0);
-      const R sg = (state & 1) ? R(0.5) : R(-0.5);
+      const R sg = (state & 1) ? R(1) : R(-1);
       R w[9];
 #pragma unroll
       for (int a = 0; a < 9; ++a) {
@@ -374,7 +374,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
         }
         const int n = a < 8 ? a : 14;
         const R xm = ld(n, o - st), x0 = ld(n, o), xp = ld(n, o + st);
-        w[a] = x0 + slope_t<R>(xm, x0, xp) * sg;
+        w[a] = x0 + half_slope_t<R>(xm, x0, xp) * sg;
       }
       const double v = flux_velocity<EOS, DIR>(A.tab, t, (double)w[0], (double)w[1], (double)w[2],
                                                (double)w[3], (double)w[4], (double)w[5],
@@ -453,11 +453,11 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     n0 = 8;
   }
   auto flux_of = [&](R vm2, R vm1, R v0, R vp1, R vp2) {
-    const R D0 = slope_t<R>(vm1, v0, vp1);
-    const R rf = vp1 - slope_t<R>(v0, vp1, vp2) * R(0.5);
-    const R lf = v0 + D0 * R(0.5);
-    const R rb = v0 - D0 * R(0.5);
-    const R lb = vm1 + slope_t<R>(vm2, vm1, v0) * R(0.5);
+    const R h0 = half_slope_t<R>(vm1, v0, vp1);
+    const R rf = vp1 - half_slope_t<R>(v0, vp1, vp2);
+    const R lf = v0 + h0;
+    const R rb = v0 - h0;
+    const R lb = vm1 + half_slope_t<R>(vm2, vm1, v0);
     return GVH_FLUXDIFF(rf, lf, rb, lb);
   };
   if constexpr (FLUX_FROM_KEPT) {
