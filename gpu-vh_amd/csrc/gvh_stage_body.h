@@ -428,12 +428,14 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     Sd2 = cdv[2] * v_s - cd(12);
   }
 
-  const R ih = R(A.inv_h[DIR]);
+  const R hih = R(A.half_inv_h[DIR]);
   const double dt = A.dt;
-  // H = (F(q_R) + F(q_L) - a (q_R - q_L))/2 with F = v q
+  // (H_b - H_f)/h with H = (F(q_R) + F(q_L) - a (q_R - q_L))/2, F = v q; the
+  // halving is applied once to the difference, (2H_b - 2H_f)/(2h), which
+  // rounds exactly as the two halved fluxes do
 #define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
-  (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) * R(0.5) -                \
-    (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf))) * R(0.5)) * ih)
+  (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) -                         \
+    (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf)))) * hih)
   // the kept states give the flux of the nine inversion variables, unless
   // LOCK == 2, which rebuilds every state from the stencil after the
   // root-finds so that none stays live through them
@@ -659,8 +661,10 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
     const double piu3 = -pinn * t2 * pr.un + pitn * pr.ut - pixn * pr.ux - piyn * pr.uy;
     const double a1 = spipi * (1.0 / sqrt(pr.e * pr.e + 3 * pr.p * pr.p));  // rho_max = 1
     const double den = 0.1 * spipi;                                          // xi0 rho_max
-    const double rho = fmax(a1, fmax(pimumu / den, fmax(piu0 / den, fmax(piu1 / den,
-                       fmax(piu2 / den, piu3 / den)))));
+    // max_i(x_i/den) as (max_i x_i)/den: a correctly rounded division by a
+    // positive den is monotone in x, so both give the same value; with
+    // den = 0 both leave rho = a1 unless some x_i > 0 (then +inf)
+    const double rho = fmax(a1, fmax(pimumu, fmax(piu0, fmax(piu1, fmax(piu2, piu3)))) / den);
     double fs = tanh(rho) / rho;
     if (fabs(rho) < 1.e-7) fs = 1.0;
     fs_rec = fs;
