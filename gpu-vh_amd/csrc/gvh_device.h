@@ -285,7 +285,7 @@ __device__ __forceinline__ void eos_p_cs2(const EosTable &T, double e, double &p
     cs2 = 0.0;
   } else if constexpr (EOS == EOS_TABLE) {
     TabPos q = tab_locate(T, e);
-    double4 v = T.pc[q.k];
+    const double4 v = ld_cell(T.pc, (unsigned)q.k);
     p = v.x + q.fr * v.y;
     cs2 = v.z + q.fr * v.w;
   } else {
@@ -331,7 +331,7 @@ __device__ __forceinline__ double eos_p(const EosTable &T, double e) {
     return e / 3.0;
   } else if constexpr (EOS == EOS_TABLE) {
     TabPos q = tab_locate(T, e);
-    double4 v = T.pc[q.k];
+    const double4 v = ld_cell(T.pc, (unsigned)q.k);
     return v.x + q.fr * v.y;
   } else {
     double p, c;
@@ -363,7 +363,7 @@ __device__ __forceinline__ void tab_p_cs2_f(const EosTable &T, float e, float &p
     k = 0;
     fr = (e - T.f_first) * T.f_inv_first_span;
   }
-  const float4 v = T.pcf[k];
+  const float4 v = ld_cell(T.pcf, (unsigned)k);
   p = v.x + fr * v.y;
   cs2 = v.z + fr * v.w;
 }
@@ -402,7 +402,7 @@ __device__ __forceinline__ double eos_T(const EosTable &T, double e) {
     return pow(e / EOS_FACTOR, 0.25);
   } else if constexpr (EOS == EOS_TABLE) {
     TabPos q = tab_locate(T, e);
-    double2 v = T.tt[q.k];
+    const double2 v = ld_cell(T.tt, (unsigned)q.k);
     return v.x + q.fr * v.y;
   } else {
     double e2 = e * e, e3 = e2 * e, e4 = e3 * e, e5 = e4 * e, e6 = e5 * e;
