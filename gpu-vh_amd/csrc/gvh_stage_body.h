@@ -495,23 +495,27 @@ struct CellPos {
 // Cell of this thread in tile b of nb tiles (a multiple of 8): consecutive
 // tiles go to the 8 XCDs in turn, so tile b is renumbered to
 // lb = (b mod 8) nb/8 + b/8 and each XCD sweeps a contiguous range.
+// The thread index and its quotients are 32-bit: a lattice's ghosted cell
+// count is below 2^29 (every array of 8-byte values is below 4 GiB), and the
+// grid exceeds the cell count by less than one strip plus 8 blocks.
 __device__ __forceinline__ CellPos cell_of_tile(const StageArgs &A, unsigned b, unsigned nb) {
   const unsigned lb = (b & 7u) * (nb >> 3) + (b >> 3);
-  const int by = A.strip;
-  const long long tid = (long long)lb * BLOCK + threadIdx.x;
-  const int i = (int)(tid % A.nx);
-  const long long r = tid / A.nx;
-  const int jj = (int)(r % by);
-  const long long r2 = r / by;
-  const int k = (int)(r2 % A.nz);
-  const int yb = (int)(r2 / A.nz);
-  const int j = yb * by + jj;
+  const unsigned by = (unsigned)A.strip, nx = (unsigned)A.nx, nz = (unsigned)A.nz;
+  const unsigned tid = lb * BLOCK + threadIdx.x;
+  const unsigned r = tid / nx;
+  const int i = (int)(tid - r * nx);
+  const unsigned r2 = r / by;
+  const int jj = (int)(r - r2 * by);
+  const unsigned yb_ = r2 / nz;
+  const int k = (int)(r2 - yb_ * nz);
+  const int yb = (int)yb_;
+  const int j = yb * (int)by + jj;
   CellPos c;
   c.valid = j < A.ny && yb < A.nstrips;
   c.i = i + NG;
   c.j = (c.valid ? j : 0) + NG;
   c.k = k + NG;
-  c.s = (unsigned)(c.i + (long long)A.ncx * (c.j + (long long)A.ncy * c.k));
+  c.s = (unsigned)c.i + (unsigned)A.ncx * ((unsigned)c.j + (unsigned)A.ncy * (unsigned)c.k);
   return c;
 }
 
