@@ -150,6 +150,67 @@ def test_kernel_arrangements_agree(monkeypatch, variant, strip):
     _check(errs)
 
 
+def limiter_edge_state(case):
+    """Plateaus (exactly equal neighbours), a jump, values alternating in
+    sign or size from cell to cell, exact zeros and negative zeros in u,
+    pi and Pi: every sign class of the limiter's differences, including
+    a = -0 (-0 - +0), where the kernels' sign test differs from the
+    reference's x < 0 only in the sign of a zero slope."""
+    nx, ny, nz = case.nx, case.ny, case.nz
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    nz0 = np.copysign(0.0, -1.0)
+    e = np.full(i.shape, 2.0)
+    e[i < 6] = 6.0
+    band = (j >= 8) & (j < 12) & (i >= 6)
+    e[band] = np.where(i[band] % 2 == 0, 3.0, 3.5)
+    ux = np.where(i < 9, 0.1, np.where(i >= 12, -0.1, 0.0))
+    ux[(i >= 9) & (i < 12) & (j < 5)] = nz0
+    uy = np.where(k % 2 == 0, 0.05 * (-1.0) ** j, 0.0)
+    un = np.zeros(i.shape)
+    un[:, :, 3] = nz0
+    ut = np.sqrt(1 + ux ** 2 + uy ** 2 + (case.t0 * un) ** 2)
+    pi10 = [np.zeros(i.shape) for _ in range(10)]
+    pi10[4] = 0.01 * e * (-1.0) ** (i + j)                       # xx, alternating
+    pi10[5] = np.where(j % 3 == 0, 0.005 * e, 0.0)               # xy, zeros ...
+    pi10[5][i < 4] = nz0                                         # ... and -0
+    pi10[7] = 0.01 * e                                           # yy, plateaus
+    Pi = -0.002 * e * (-1.0) ** k
+    return e, case.pressure(e), ut, ux, uy, un, pi10, Pi
+
+
+def test_limiter_edge_states():
+    """FP64: the edge-case state (plateaus, a jump, sign alternation, +-0)
+    after 5 steps agrees with the oracle to 1e-10 per cell.  Mixed: e and
+    u^mu within 1e-4 per cell (1% floor), pi within 1e-3."""
+    case = Case(19, 17, 7, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=gpuvh.EOS_TABLE,
+                zeta_tpeak=0.16 / gpuvh.HBARC)
+    state = limiter_edge_state(case)
+    fields = [gpuvh.E] + U_FIELDS + PI_FIELDS + [gpuvh.PI]
+    orc, ctx = run_both(case, state, 5)
+    rep = []
+    errs = compare(orc, ctx, fields, tau=0.5 + 5 * 0.01, report=rep)
+    print("limiter edge state, fp64", errs)
+    _check(errs, report=rep)
+    assert ctx.diagnostics()["e_nan"] == 0
+    ctx.close()
+    mix = case.context(gpuvh.MIXED)
+    mix.set_initial_state(case.t0, *state)
+    mix.steps(case.t0, 0, 5)
+    mix.synchronize()
+    from parity import floor_scales
+    refs = {f: orc.get(f) for f in fields}
+    scales = floor_scales(refs, 0.5 + 5 * 0.01)
+    merr = {gpuvh.FIELD_NAMES[f]: rel_err(mix.get(f), refs[f], floor_frac=1e-2, scale=scales[f])
+            for f in fields}
+    print("limiter edge state, mixed", merr)
+    for f in [gpuvh.E] + U_FIELDS:
+        assert merr[gpuvh.FIELD_NAMES[f]] <= 1e-4, gpuvh.FIELD_NAMES[f]
+    for f in PI_FIELDS + [gpuvh.PI]:
+        assert merr[gpuvh.FIELD_NAMES[f]] <= 1e-3, gpuvh.FIELD_NAMES[f]
+    mix.close()
+    orc.close()
+
+
 def test_full_size_benchmark_event():
     """The benchmark's own event at its full size: 201x201x101, dx = deta =
     0.1, tau0 = 0.5, dtau = 0.01, tabulated EoS, eta/s = 0.16, zeta/s peak

@@ -527,3 +527,4 @@ def test_regulation_factor_closed_form():
 def gpuvh_path(rel):
     import os
     return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), rel)
+
