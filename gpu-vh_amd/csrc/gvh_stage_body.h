@@ -27,7 +27,7 @@
 // Waves per SIMD asked of the one-event stage kernel (the second launch
 // bound is amdgpu_waves_per_eu; 1 leaves the register allocation free).
 // Measured on the 201x201x101 table-EoS event: the FP32 (mixed) form at three
-// waves with a few spills runs 13% faster than at its free two; the FP64 form
+// waves (166-168 VGPRs) runs 13% faster than at its free two; the FP64 form
 // runs 5% slower when held to two than when free (same register count,
 // different schedule), and at three its spills cost more than the wave gains.
 #ifndef GVH_WAVES_FP32
