@@ -153,7 +153,8 @@ def mixed_error(args, e, stream, nsteps):
         out[name] = worst
     ut = ctxs[gpuvh.FP64].get(gpuvh.UT)
     out["u_lorentz"] = du / float(np.nanmin(np.abs(ut)))
-    # the north-star criterion: the fluid at or above the freeze-out energy
+    # the mixed form's accuracy criterion (1e-4 in e and u^mu at freeze-out):
+    # the fluid at or above the freeze-out energy
     # density (T >= 0.15 GeV); and where the global errors come from, by the
     # FP64 energy density of the cell
     e_fo = ctxs[gpuvh.FP64].energy_density_of_T(0.15 / gpuvh.HBARC)

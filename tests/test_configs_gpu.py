@@ -12,10 +12,11 @@ device against the oracle (the FP64 CPU restatement of the reference KT step):
 - the serving loop: events of the serving workload arriving at Poisson times
   while others run, each result equal to the same event run alone.
 
-Mixed-form errors are measured where the north-star criterion puts them, on
-the fluid at or above the freeze-out energy density e_fo (T >= 0.15 GeV):
-e relative per cell, u^mu as max_mu |du^mu| / u^tau (u^eta scaled by tau),
-pi^{mu nu} relative to the largest |pi| there.
+Mixed-form errors are measured where the mixed form's accuracy criterion
+(1e-4 in e and u^mu at freeze-out) puts them, on the fluid at or above the
+freeze-out energy density e_fo (T >= 0.15 GeV): e relative per cell, u^mu as
+max_mu |du^mu| / u^tau (u^eta scaled by tau), pi^{mu nu} relative to the
+largest |pi| there.
 """
 import numpy as np
 import pytest
