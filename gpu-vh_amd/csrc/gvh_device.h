@@ -246,6 +246,12 @@ __device__ __forceinline__ float rcp_t(float b) { return __builtin_amdgcn_rcpf(b
 __device__ __forceinline__ float div_t(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float sqrt_t(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// a / b in the stencil and source arithmetic of a stage: the IEEE division
+// in FP64 (the reference's expression), the hardware reciprocal (about 1 ulp)
+// times a in FP32.
+__device__ __forceinline__ double div_st(double a, double b) { return a / b; }
+__device__ __forceinline__ float div_st(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // ---------------------------------------------------------------- EoS
 struct TabPos {
   int k;

@@ -63,13 +63,13 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
                                              R e, R p, R ut, R ux, R uy, R un, R utp, R uxp,
                                              R uyp, R unp, const GradT<R> &g, R *S) {
   const R t = R(A.t);
-  const R inv_t = R(1.0) / t;
+  const R inv_t = div_st(R(1.0), t);
   const R t2 = t * t, t3 = t * t2;
-  const R inv_t2 = R(1.0) / t2;
+  const R inv_t2 = div_st(R(1.0), t2);
   const R Pi = BULK ? Q[14] : R(0.0);
   const R pitt = Q[4], pitx = Q[5], pity = Q[6], pitn = Q[7], pixx = Q[8],
                pixy = Q[9], pixn = Q[10], piyy = Q[11], piyn = Q[12], pinn = Q[13];
-  const R inv_ut = R(1.0) / ut;
+  const R inv_ut = div_st(R(1.0), ut);
 
   // T^{tau mu} sources, SourceTerms.cu:726-737
   const R PP = p + Pi;
@@ -86,7 +86,7 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
   // transport coefficients, SourceTerms.cu:46-60
   const double T_eos = eos_T<EOS>(A.tab, e_eos);
   const R T = R(T_eos);
-  const R taupiInv = R(0.2f) * (T / R(A.etabar));
+  const R taupiInv = R(0.2f) * div_st(T, R(A.etabar));
   const R beta_pi = R(0.2f) * (e + p);
 
   // backward-difference time derivative and covariant acceleration, :67-86
@@ -120,7 +120,7 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
   const R syy = -(g.uy[1] + uy * duy) + (R(1.0) + uy * uy) * th3;
   const R syn = -ut * uy * un * inv_t - (g.un[1] + g.uy[2] * inv_t2) * R(0.5) -
                      (un * duy + uy * dun) * R(0.5) + uy * un * th3;
-  const R snn = -ut * (R(1.0) + R(2.0) * t2 * un2) / t3 - g.un[2] * inv_t2 - un * dun +
+  const R snn = div_st(-ut * (R(1.0) + R(2.0) * t2 * un2), t3) - g.un[2] * inv_t2 - un * dun +
                      (inv_t2 + un2) * th3;
 
   // omega^mu_nu, :123-136
@@ -253,7 +253,7 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
     const R a2 = a * a;
     const R beta_Pi = R(15.0) * a2 * (e + p);
     const R lambda_Pipi = R(1.6f) * a;
-    const R tauPiInv = R(15.0) * a2 * (T / R(zeta_over_s(T_eos, A.zeta_tpeak)));
+    const R tauPiInv = R(15.0) * a2 * div_st(T, R(zeta_over_s(T_eos, A.zeta_tpeak)));
     constexpr R d_PP = R(0.666667);  // TransportCoefficients.cuh:18
     const R dPi = -beta_Pi * theta - Pi * tauPiInv - d_PP * Pi * theta + lambda_Pipi * ps;
     S[14] = dPi * inv_ut + Pi * dkvk;
@@ -423,7 +423,7 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
     Sd3 = cdv[3] * v_s - cd(12);
   } else {
     Sd0 = cdv[0] * v_s - cdv[3] - (BULK ? v_s * cPi : R(0));
-    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? cPi / R(t * t) : R(0));
+    Sd3 = cdv[3] * v_s - cd(13) - (BULK ? div_st(cPi, R(t * t)) : R(0));
     Sd1 = cdv[1] * v_s - cd(10);
     Sd2 = cdv[2] * v_s - cd(12);
   }
@@ -626,9 +626,9 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
   }
 
   // flux velocity of the cell itself for the directional sources (v = u/u^tau)
-  direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, R(ux / ut), acc);
-  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS>(A, s, e_s, R(uy / ut), acc);
-  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, R(un / ut), acc);
+  direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(ux), R(ut)), acc);
+  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(uy), R(ut)), acc);
+  if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(un), R(ut)), acc);
 
   if constexpr (STAGE2) {
     // q^n of the convex average: all NV loads in flight together
