@@ -430,12 +430,15 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
 
   const R hih = R(A.half_inv_h[DIR]);
   const double dt = A.dt;
-  // (H_b - H_f)/h with H = (F(q_R) + F(q_L) - a (q_R - q_L))/2, F = v q; the
-  // halving is applied once to the difference, (2H_b - 2H_f)/(2h), which
-  // rounds exactly as the two halved fluxes do
+  // (H_b - H_f)/h with H = (F(q_R) + F(q_L) - a (q_R - q_L))/2, F = v q,
+  // written per interface as 2H = (v_R - a) q_R + (v_L + a) q_L: the two
+  // weights are formed once per direction, so each variable's flux is one
+  // multiply and one FMA (equal to the reference's expression up to
+  // rounding).  The halving is applied once to the difference,
+  // (2H_b - 2H_f)/(2h).
+  const R wRb = rRb - aB, wLb = rLb + aB, wRf = rRf - aF, wLf = rLf + aF;
 #define GVH_FLUXDIFF(rf, lf, rb, lb)                                         \
-  (((rRb * (rb) + rLb * (lb) - aB * ((rb) - (lb))) -                         \
-    (rRf * (rf) + rLf * (lf) - aF * ((rf) - (lf)))) * hih)
+  (((wRb * (rb) + wLb * (lb)) - (wRf * (rf) + wLf * (lf))) * hih)
   // the kept states give the flux of the nine inversion variables, unless
   // LOCK == 2, which rebuilds every state from the stencil after the
   // root-finds so that none stays live through them
