@@ -436,6 +436,14 @@ __device__ __forceinline__ double zeta_over_s(double T, double tpeak) {
   if (x < 0.995) return 0.9 * exp((x - 1) / 0.0025) + 0.22 * exp((x - 1) / 0.022) + 0.03;
   return -13.77 * x * x + 27.55 * x - 13.45;
 }
+// The same in FP32 for the mixed form's source terms: divisions by the
+// constant widths as products with their reciprocals, the hardware exp.
+__device__ __forceinline__ float zeta_over_s(float T, float tpeak) {
+  const float x = T * __builtin_amdgcn_rcpf(tpeak), d = x - 1.0f;
+  if (x > 1.05f) return 0.9f * __expf(d * -40.0f) + 0.25f * __expf(d * (-1.0f / 0.13f)) + 0.001f;
+  if (x < 0.995f) return 0.9f * __expf(d * 400.0f) + 0.22f * __expf(d * (1.0f / 0.022f)) + 0.03f;
+  return -13.77f * x * x + 27.55f * x - 13.45f;
+}
 
 // ------------------------------------------------------------ root-find
 // energyDensityFromConservedVariables, EnergyMomentumTensor.cu:19-48:

@@ -246,14 +246,17 @@ __device__ __forceinline__ void source_terms(const StageArgs &A, const R *Q, dou
     S[13] = r * inv_ut + pinn * dkvk;
   }
   if constexpr (BULK) {
-    // bulk relaxation, :53-60 and :238, :254
+    // bulk relaxation, :53-60 and :238, :254; c_s^2 from the FP64 EoS (1/3 -
+    // c_s^2 is small where the fluid is nearly conformal), zeta/s in the
+    // arithmetic of the source terms
+    using FE = R;
     double p_, cs2;
     eos_p_cs2<EOS>(A.tab, e_eos, p_, cs2);
     const R a = R(0.333333f) - R(cs2);
     const R a2 = a * a;
     const R beta_Pi = R(15.0) * a2 * (e + p);
     const R lambda_Pipi = R(1.6f) * a;
-    const R tauPiInv = R(15.0) * a2 * div_st(T, R(zeta_over_s(T_eos, A.zeta_tpeak)));
+    const R tauPiInv = R(15.0) * a2 * div_st(T, R(zeta_over_s(FE(T_eos), FE(A.zeta_tpeak))));
     constexpr R d_PP = R(0.666667);  // TransportCoefficients.cuh:18
     const R dPi = -beta_Pi * theta - Pi * tauPiInv - d_PP * Pi * theta + lambda_Pipi * ps;
     S[14] = dPi * inv_ut + Pi * dkvk;
