@@ -209,9 +209,11 @@ __device__ __forceinline__ double slope(double x, double y, double z) {
 // ------------------------------------------------ division, square root
 // The IEEE-exact expansions of a/b and sqrt(x) on gfx950 cost ~11 and ~15
 // VALU instructions.  These start from the hardware approximations
-// (v_rcp_f64, v_rsq_f64, about 2^-26 relative) and refine with FMA Newton
-// steps to within about one ulp.  Operands here are finite, normal and
-// nonzero in any physical state; special values are not treated.
+// (v_rcp_f64, v_rsq_f64: at most 4.6e-8 and 5.3e-8 relative, measured on
+// 4M inputs over [1e-6, 1e6]) and refine with FMA steps to 0.5 ulp on
+// those inputs (the reciprocal needs both Newton steps: one leaves 10 ulp).
+// Operands here are finite, normal and nonzero in any physical state;
+// special values are not treated.
 __device__ __forceinline__ double rcp_nr(double b) {
   double r = __builtin_amdgcn_rcp(b);
   double e = fma(-b, r, 1.0);
