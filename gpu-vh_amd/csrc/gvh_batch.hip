@@ -115,7 +115,7 @@ size_t event_footprint(const gvh_lattice_parameters &L, int precision, int surfa
   size_t bytes = (3 * qstride + 16 * G) * sizeof(double) + (1u << 20);
   if (surface) {
     const size_t nrow = (size_t)fofreq * nx * ny;
-    bytes += (size_t)(fofreq + 1) * 10 * nint * sizeof(double);  // ring
+    bytes += 2 * (size_t)(fofreq + 1) * 10 * nint * sizeof(double);  // ring, two halves
     bytes += 2 * nrow * sizeof(unsigned) * 5 / 4;                // row counts and offsets
     bytes += nint / 20 * (3 * sizeof(unsigned) + 80 * sizeof(double) + 200 * sizeof(double));
   }
@@ -137,10 +137,23 @@ struct gvh_batch {
   // priority, like every stream here)
   static constexpr int NSEARCH = 8;
   hipStream_t search[NSEARCH] = {};
-  // pinned staging of the rows of each due window (device -> host at full
-  // PCIe rate), then copied into the event's row blocks by host threads
-  std::vector<double *> stage;
-  std::vector<size_t> stage_cap;  // doubles
+  // the window search of each slot's event: phase 1 (cube counts), 2
+  // (elements), 3 (rows copied to the slot's pinned staging, then into the
+  // event's row blocks by host threads).  One phase is issued per step once
+  // the previous one has finished, so a search runs beside the next steps:
+  // the window it reads is one ring half, and those steps record into the
+  // other.  It is completed before a step would record into its half again,
+  // before the event's next window is searched, and before the event retires.
+  struct Search {
+    int phase = 0;
+    int n = 0, m = 0;   // loop index and window of the search
+    double t = 0;       // the loop's time at that window
+    long long ne = 0;   // rows
+    hipEvent_t ev = nullptr;  // recorded after the phase's device-to-host copy
+    double *stage = nullptr;
+    size_t cap = 0;     // doubles
+  };
+  std::vector<Search> srch;
   size_t budget = 0, used = 0;
   int max_slots = 0, fofreq = 10, autotune = 0;
   BatchSlot *d_slots = nullptr;
@@ -191,6 +204,78 @@ void parallel_copy(const std::vector<std::pair<double *, const double *>> &jobs,
   for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
   work();
   for (auto &t : pool) t.join();
+}
+
+hipStream_t search_stream(gvh_batch *b, int s) { return b->search[s % gvh_batch::NSEARCH]; }
+
+// Issue the next phase of slot s's window search once the previous phase
+// has finished: with block, waiting for it; otherwise only if it has.  A
+// finished third phase appends the window's rows to the event's row blocks.
+int advance_search(gvh_batch *b, int s, bool block) {
+  gvh_batch::Search &W = b->srch[s];
+  if (W.phase == 0) return GVH_OK;
+  if (block) {
+    HIP_TRY(hipEventSynchronize(W.ev));
+  } else {
+    const hipError_t q = hipEventQuery(W.ev);
+    if (q == hipErrorNotReady) return GVH_OK;
+    if (q != hipSuccess) return gvh_fail(GVH_EHIP, "window search: %s", hipGetErrorString(q));
+  }
+  Live &L = b->live[s];
+  const hipStream_t st = search_stream(b, s);
+  int rc;
+  if (W.phase == 1) {
+    if ((rc = fo_find_elements(L.ctx, st))) return rc;
+    HIP_TRY(hipEventRecord(W.ev, st));
+    W.phase = 2;
+  } else if (W.phase == 2) {
+    if ((rc = fo_find_emit(L.ctx, W.t, st, &W.ne))) return rc;
+    const size_t need = (size_t)W.ne * GVH_FO_COLUMNS;
+    if (need > W.cap) {
+      if (W.stage) HIP_TRY(hipHostFree(W.stage));
+      W.stage = nullptr;
+      W.cap = 0;
+      const size_t cap = need + need / 2;
+      HIP_TRY(hipHostMalloc((void **)&W.stage, cap * sizeof(double)));
+      W.cap = cap;
+    }
+    if (W.ne > 0 && (rc = fo_rows_async(L.ctx, W.stage, W.ne, st))) return rc;
+    HIP_TRY(hipEventRecord(W.ev, st));
+    W.phase = 3;
+  } else {
+    if (W.ne > 0)
+      parallel_copy({{L.rows.append(W.ne), W.stage}}, {(size_t)W.ne * GVH_FO_COLUMNS});
+    W.phase = 0;
+  }
+  return GVH_OK;
+}
+
+int complete_search(gvh_batch *b, int s) {
+  int rc;
+  while (b->srch[s].phase)
+    if ((rc = advance_search(b, s, true))) return rc;
+  return GVH_OK;
+}
+
+// Start the search of the window due at loop index n of slot s's event; it
+// reads ring half (n / fofreq) mod 2 once this step's record graph has run.
+int start_search(gvh_batch *b, int s, int n) {
+  gvh_batch::Search &W = b->srch[s];
+  Live &L = b->live[s];
+  int rc;
+  if ((rc = complete_search(b, s))) return rc;
+  if (!W.ev) HIP_TRY(hipEventCreateWithFlags(&W.ev, hipEventDisableTiming));
+  const hipStream_t st = search_stream(b, s);
+  W.n = n;
+  W.m = n / b->fofreq;
+  W.t = L.ctx->Hp.initialProperTimePoint + L.steps * L.ctx->L.latticeSpacingProperTime;
+  W.ne = 0;
+  if ((rc = fo_set_window_half(L.ctx, W.m & 1))) return rc;
+  HIP_TRY(hipStreamWaitEvent(st, b->recorded, 0));
+  if ((rc = fo_find_count(L.ctx, n, st))) return rc;
+  HIP_TRY(hipEventRecord(W.ev, st));
+  W.phase = 1;
+  return GVH_OK;
 }
 
 int grow_tiles(gvh_batch *b, size_t n) {
@@ -406,7 +491,7 @@ int admit(gvh_batch *b) {
       }
     }
     double e_fo = host_e_of_T(c->eos, P.H.freezeoutTemperatureGeV / 0.197326938);
-    if (!rc && P.surface) rc = gvh_freezeout_init(c, b->fofreq, e_fo, P.fo_flags);
+    if (!rc && P.surface) rc = fo_init(c, b->fofreq, e_fo, P.fo_flags, 2);
     if (rc) {
       gvh_destroy(c);
       fail_pending(b, P, rc);
@@ -427,6 +512,7 @@ int admit(gvh_batch *b) {
     S.bulk = c->bulk ? 1 : 0;
     S.e_fo = e_fo;
     S.ring = P.surface ? fo_ring(c->fo) : nullptr;
+    S.halves = 2;
     S.nint = (long long)c->nx * c->ny * c->nz;
     Live &L = b->live[s];
     L = Live();
@@ -453,6 +539,8 @@ int admit(gvh_batch *b) {
 }
 
 int retire(gvh_batch *b, int s) {
+  int rc0 = complete_search(b, s);
+  if (rc0) return rc0;
   Live &L = b->live[s];
   const BatchSlot &S = b->h_slots[s];
   gvh_context *c = L.ctx;
@@ -557,6 +645,7 @@ extern "C" int gvh_batch_create(const gvh_options *opts, const gvh_batch_options
   }
   b->budget = B.hbm_bytes ? (size_t)B.hbm_bytes : free_b / 10 * 8;
   b->live.resize(b->max_slots);
+  b->srch.resize(b->max_slots);
   *out = b;
   return GVH_OK;
 }
@@ -577,8 +666,10 @@ extern "C" int gvh_batch_destroy(gvh_batch *b) {
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (hipStream_t &q : b->search)
     if (q) (void)hipStreamDestroy(q);
-  for (double *p : b->stage)
-    if (p) (void)hipHostFree(p);
+  for (gvh_batch::Search &W : b->srch) {
+    if (W.stage) (void)hipHostFree(W.stage);
+    if (W.ev) (void)hipEventDestroy(W.ev);
+  }
   if (b->recorded) (void)hipEventDestroy(b->recorded);
   delete b;
   return GVH_OK;
@@ -628,66 +719,31 @@ extern "C" int gvh_batch_step(gvh_batch *b, int *nrunning) {
     return GVH_OK;
   }
   // the windows due at this step (loop index n = steps before it + 1): their
-  // history is complete once the record graph has run; they are searched
-  // together, phase by phase, on the search streams while the stages run
+  // history is complete once the record graph has run.  A search still in
+  // flight is completed first if this step records into its ring half again
+  // (window m + 2) or its event has a new window due.
   std::vector<int> due;
   for (int s = 0; s < b->max_slots; ++s) {
     const Live &L = b->live[s];
+    if (!L.ctx) continue;
     const int n = L.steps + 1;
-    if (L.ctx && L.surface && n <= L.max_steps && n % b->fofreq == b->fofreq - 1) due.push_back(s);
+    const bool is_due = L.surface && n <= L.max_steps && n % b->fofreq == b->fofreq - 1;
+    if (is_due) due.push_back(s);
+    if (b->srch[s].phase && (is_due || n / b->fofreq >= b->srch[s].m + 2) &&
+        (rc = complete_search(b, s)))
+      return rc;
   }
   HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   if (!due.empty()) HIP_TRY(hipEventRecord(b->recorded, b->stream));
   HIP_TRY(hipGraphLaunch(b->exec2, b->stream));
   HIP_TRY(hipMemcpyAsync(b->h_slots, b->d_slots, b->max_slots * sizeof(BatchSlot),
                          hipMemcpyDeviceToHost, b->stream));
-  if (!due.empty()) {
-    auto qs = [&](size_t k) { return b->search[k % gvh_batch::NSEARCH]; };
-    auto wait_all = [&]() -> int {
-      for (size_t k = 0; k < due.size() && k < (size_t)gvh_batch::NSEARCH; ++k)
-        HIP_TRY(hipStreamSynchronize(qs(k)));
-      return GVH_OK;
-    };
-    for (size_t k = 0; k < due.size() && k < (size_t)gvh_batch::NSEARCH; ++k)
-      HIP_TRY(hipStreamWaitEvent(qs(k), b->recorded, 0));
-    for (size_t k = 0; k < due.size(); ++k)
-      if ((rc = fo_find_count(b->live[due[k]].ctx, b->live[due[k]].steps + 1, qs(k)))) return rc;
-    if ((rc = wait_all())) return rc;
-    for (size_t k = 0; k < due.size(); ++k)
-      if ((rc = fo_find_elements(b->live[due[k]].ctx, qs(k)))) return rc;
-    if ((rc = wait_all())) return rc;
-    if (b->stage.size() < due.size()) {
-      b->stage.resize(due.size(), nullptr);
-      b->stage_cap.resize(due.size(), 0);
-    }
-    std::vector<long long> ne(due.size(), 0);
-    for (size_t k = 0; k < due.size(); ++k) {
-      Live &L = b->live[due[k]];
-      // (the slot mirror is being copied back meanwhile: t from the context)
-      const double t = L.ctx->Hp.initialProperTimePoint + L.steps * L.ctx->L.latticeSpacingProperTime;
-      if ((rc = fo_find_emit(L.ctx, t, qs(k), &ne[k]))) return rc;
-      const size_t need = (size_t)ne[k] * GVH_FO_COLUMNS;
-      if (need > b->stage_cap[k]) {
-        if (b->stage[k]) HIP_TRY(hipHostFree(b->stage[k]));
-        b->stage[k] = nullptr;
-        b->stage_cap[k] = 0;
-        const size_t cap = need + need / 2;
-        HIP_TRY(hipHostMalloc((void **)&b->stage[k], cap * sizeof(double)));
-        b->stage_cap[k] = cap;
-      }
-      if (ne[k] > 0 && (rc = fo_rows_async(L.ctx, b->stage[k], ne[k], qs(k)))) return rc;
-    }
-    if ((rc = wait_all())) return rc;
-    // staging -> the events' row blocks, split over host threads
-    std::vector<std::pair<double *, const double *>> jobs;
-    std::vector<size_t> len;
-    for (size_t k = 0; k < due.size(); ++k)
-      if (ne[k] > 0) {
-        jobs.emplace_back(b->live[due[k]].rows.append(ne[k]), b->stage[k]);
-        len.push_back((size_t)ne[k] * GVH_FO_COLUMNS);
-      }
-    parallel_copy(jobs, len);
-  }
+  // searches in flight move on by a phase if the previous one has finished;
+  // the due windows start theirs beside the stages
+  for (int s = 0; s < b->max_slots; ++s)
+    if (b->srch[s].phase && (rc = advance_search(b, s, false))) return rc;
+  for (int s : due)
+    if ((rc = start_search(b, s, b->live[s].steps + 1))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   running = 0;
   for (int s = 0; s < b->max_slots; ++s) {

@@ -59,10 +59,15 @@ __global__ __launch_bounds__(BLOCK) void batch_record_kernel(BatchSlot *__restri
       const long long cell =
           (c.i - NG) + (long long)S.A.nx * ((c.j - NG) + (long long)S.A.ny * (c.k - NG));
       const int n = S.steps + 1, nfo = n % S.fofreq;
-      double *ring = S.ring;
+      // window m = n / fofreq records into half m mod 2 when the ring has two
+      // halves; its slot 0 is the previous window's last slot
+      const int m = n / S.fofreq;
+      const long long span = (long long)(S.fofreq + 1) * 10 * nint;
+      double *ring = S.ring + (S.halves == 2 && (m & 1) ? span : 0);
+      const double *prev = S.ring + (S.halves == 2 && m > 0 && ((m - 1) & 1) ? span : 0);
       if (nfo == 0)
         for (int v = 0; v < 10; ++v)
-          ring[(long long)v * nint + cell] = ring[((long long)S.fofreq * 10 + v) * nint + cell];
+          ring[(long long)v * nint + cell] = prev[((long long)S.fofreq * 10 + v) * nint + cell];
       double *dst = ring + (long long)(nfo == 0 ? 1 : nfo + 1) * 10 * nint + cell;
       dst[0 * nint] = u[1 * G + s];
       dst[1 * nint] = u[2 * G + s];

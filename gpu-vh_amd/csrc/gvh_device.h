@@ -140,8 +140,9 @@ struct BatchSlot {
   int tile_first, ntiles;  // its tiles (blocks of cells) in the tile map
   double e_fo;
   unsigned long long above;  // interior cells with e > e_fo in the current state
-  double *ring;       // freeze-out history, (fofreq+1) x 10 x nint, or nullptr
+  double *ring;       // freeze-out history, halves x (fofreq+1) x 10 x nint, or nullptr
   long long nint;
+  int halves;         // 2: window m records into ring half m mod 2
 };
 
 // ------------------------------------------------------- cell access

@@ -64,6 +64,7 @@ namespace gvh {
 
 struct FoState {
   int dim = 0, fofreq = 0, flags = 0;
+  int halves = 1;  // ring copies of fofreq + 1 slots (2: a batch alternates them by window)
   double e_fo = 0;
   long long nint = 0;
   double *ring = nullptr;
@@ -591,7 +592,17 @@ unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / 
 // ================================================================= C ABI
 
 extern "C" int gvh_freezeout_init(gvh_context *c, int fofreq, double e_fo, int flags) {
+  return fo_init(c, fofreq, e_fo, flags, 1);
+}
+
+// The history of a context with `halves` copies of the fofreq + 1 slots: one
+// for a context's own run loop, two for a batched event, whose consecutive
+// windows alternate between them so that a window can be searched while the
+// next window's steps record into the other half.
+namespace gvh {
+int fo_init(gvh_context *c, int fofreq, double e_fo, int flags, int halves) {
   if (!c) return gvh_fail(GVH_EINVAL, "context is NULL");
+  if (halves != 1 && halves != 2) return gvh_fail(GVH_EINVAL, "halves must be 1 or 2");
   if (fofreq < 3 || fofreq > GVH_FO_MAX_FREQ)
     return gvh_fail(GVH_EINVAL, "fofreq must be in [3, %d], got %d", GVH_FO_MAX_FREQ, fofreq);
   int dim;
@@ -612,9 +623,10 @@ extern "C" int gvh_freezeout_init(gvh_context *c, int fofreq, double e_fo, int f
   f->flags = flags;
   f->e_fo = e_fo;
   f->nint = nint;
+  f->halves = halves;
   f->slot.resize(fofreq + 1);
   for (int k = 0; k <= fofreq; ++k) f->slot[k] = k;
-  const size_t bytes = (size_t)(fofreq + 1) * NVAR * nint * sizeof(double);
+  const size_t bytes = (size_t)halves * (fofreq + 1) * NVAR * nint * sizeof(double);
   hipError_t e = hipMalloc((void **)&f->ring, bytes);
   if (e == hipSuccess) e = hipMemsetAsync(f->ring, 0, bytes, c->stream);
   if (e == hipSuccess) e = hipMalloc((void **)&f->total, 2 * sizeof(unsigned));
@@ -625,6 +637,16 @@ extern "C" int gvh_freezeout_init(gvh_context *c, int fofreq, double e_fo, int f
   c->fo = f;
   return GVH_OK;
 }
+
+// The next window search reads ring half `half` (slots half (fofreq + 1) ..
+// half (fofreq + 1) + fofreq), as a batch's record kernel wrote them.
+int fo_set_window_half(gvh_context *c, int half) {
+  FoState *f = c->fo;
+  if (!f || half < 0 || half >= f->halves) return gvh_fail(GVH_EINVAL, "no ring half %d", half);
+  for (int k = 0; k <= f->fofreq; ++k) f->slot[k] = half * (f->fofreq + 1) + k;
+  return GVH_OK;
+}
+}  // namespace gvh
 
 // The reference's per-step update of the history (HydroPlugin.cpp:260-269,
 // FreezeOut.cpp:49-126) for loop index n >= 1.

@@ -48,6 +48,8 @@ Roles context_roles(const gvh_context *c);
 double *fo_ring(FoState *f);
 // The window search in phases (gvh_freezeout.hip): each phase may start once
 // the previous one's stream work is complete.
+int fo_init(gvh_context *c, int fofreq, double e_fo, int flags, int halves);
+int fo_set_window_half(gvh_context *c, int half);
 int fo_find_count(gvh_context *c, int n, hipStream_t st);
 int fo_find_elements(gvh_context *c, hipStream_t st);
 int fo_find_emit(gvh_context *c, double t, hipStream_t st, long long *nelements);
