@@ -400,8 +400,10 @@ int gvh_batch_destroy(gvh_batch *b);
  * status; the other events go on. */
 int gvh_batch_submit(gvh_batch *b, const gvh_event *ev, long long *id);
 /* Admit queued events that fit, advance every running event by one step,
- * search due freeze-out windows, retire finished events.  *nrunning =
- * running + queued events afterwards. */
+ * move the freeze-out window searches on, retire finished events.  A
+ * window's search runs beside the event's next steps (its history ring has
+ * two halves, alternating by window); its rows are in gvh_batch_rows once the
+ * event has finished.  *nrunning = running + queued events afterwards. */
 int gvh_batch_step(gvh_batch *b, int *nrunning);
 /* gvh_batch_step until nothing is queued or running. */
 int gvh_batch_run(gvh_batch *b);
