@@ -46,7 +46,7 @@ def make_events(specs, table):
     return out
 
 
-def run_alone(opts, L, H, e, surface, fo_flags=0):
+def run_alone(opts, L, H, e, surface, fo_flags=0, fofreq=FOFREQ):
     """The run loop's stepping, freeze-out windows and stops on one context."""
     ctx = gpuvh.Context(L, H, opts)
     shape = e.shape
@@ -54,17 +54,17 @@ def run_alone(opts, L, H, e, surface, fo_flags=0):
     ctx.set_initial_state(H.initialProperTimePoint, e, ctx.equilibrium_pressure(e), ut, ux, uy, un)
     e_fo = ctx.energy_density_of_T(H.freezeoutTemperatureGeV / gpuvh.HBARC)
     if surface:
-        ctx.freezeout_init(FOFREQ, e_fo, fo_flags)
+        ctx.freezeout_init(fofreq, e_fo, fo_flags)
     t0, dt, nt = H.initialProperTimePoint, L.latticeSpacingProperTime, L.numProperTimePoints
     rows, t, steps, frozen_count, frozen = [], t0, 0, 0, False
     for n in range(1, nt + 2):
         if surface:
             ctx.freezeout_record(n)
-            if n % FOFREQ == FOFREQ - 1:
+            if n % fofreq == fofreq - 1:
                 rows.append(ctx.freezeout_find(n, t))
         if ctx.count_above(e_fo) == 0:
             frozen_count += 1
-        if frozen_count >= FOFREQ + 1:
+        if frozen_count >= fofreq + 1:
             frozen = True
             break
         ctx.step(t)
@@ -78,9 +78,9 @@ FIELDS = [gpuvh.E, gpuvh.UT, gpuvh.UX, gpuvh.UY, gpuvh.UN, gpuvh.PIXX, gpuvh.PIX
           gpuvh.PI]
 
 
-def run_batch(opts, events, surface, hbm_bytes=0):
+def run_batch(opts, events, surface, hbm_bytes=0, fofreq=FOFREQ):
     """(result, rows, {field: array}) per event, in submission order."""
-    batch = gpuvh.Batch(opts, hbm_bytes=hbm_bytes, max_events=8, autotune=False)
+    batch = gpuvh.Batch(opts, hbm_bytes=hbm_bytes, max_events=8, autotune=False, fofreq=fofreq)
     ids = [batch.submit(L, H, e, surface=surface, keep_state=True) for (L, H, e) in events]
     batch.run()
     res = {r["id"]: r for r in batch.poll()}
@@ -101,12 +101,12 @@ def assert_same(a, b, label):
         np.testing.assert_array_equal(fa[f], fb[f], err_msg="%s %s" % (label, gpuvh.FIELD_NAMES[f]))
 
 
-def assert_close_to_context(res, opts, ev, label):
+def assert_close_to_context(res, opts, ev, label, fofreq=FOFREQ):
     """Against the run loop on a context of its own: same steps and stop,
     fields to 1e-10 (floor 1e-4 of the field's maximum), rows to 1e-8."""
     r, rows, fields = res
     L, H, e = ev
-    ctx, rows_c, steps, frozen = run_alone(opts, L, H, e, True)
+    ctx, rows_c, steps, frozen = run_alone(opts, L, H, e, True, fofreq=fofreq)
     say("  ", label, "steps", r["steps"], "context", steps, "rows", len(rows), len(rows_c))
     assert r["steps"] == steps and bool(r["frozen"]) == frozen, label
     for f in FIELDS:
@@ -121,6 +121,25 @@ def assert_close_to_context(res, opts, ev, label):
     if len(rows):
         np.testing.assert_allclose(rows, rows_c, rtol=1e-8, atol=1e-12 * np.abs(rows_c).max())
     ctx.close()
+
+
+def test_windows_every_three_steps(monkeypatch):
+    """fofreq 3, the smallest: a window's search overlaps the next steps and
+    must be finished before the step that records into its ring half again
+    (four steps later).  Four events of different shapes and lengths in one
+    batch: steps, stop and rows as the run loop on contexts of their own
+    (rows to 1e-8, fields to 1e-10), and bitwise as each event alone."""
+    monkeypatch.setenv("GVH_VARIANT", "0")
+    table = gpuvh.EosTable()
+    opts = gpuvh.options(gpuvh.EOS_TABLE, True, gpuvh.FP64, 0.16 / gpuvh.HBARC, table=table)
+    specs = [(31, 29, 1, 0.2, 0.1, 0.02, 23, 11), (21, 25, 1, 0.25, 0.1, 0.025, 17, 12),
+             (13, 15, 9, 0.3, 0.3, 0.03, 20, 13), (11, 11, 7, 0.3, 0.3, 0.03, 14, 14)]
+    events = make_events(specs, table)
+    mix = run_batch(opts, events, True, fofreq=3)
+    for k, ev in enumerate(events):
+        assert len(mix[k][1]) > 0, "event %d has no surface rows" % k
+        assert_close_to_context(mix[k], opts, ev, "event %d" % k, fofreq=3)
+        assert_same(mix[k], run_batch(opts, [ev], True, fofreq=3)[0], "event %d" % k)
 
 
 @pytest.mark.parametrize("precision", [gpuvh.FP64, gpuvh.MIXED], ids=["fp64", "mixed"])
