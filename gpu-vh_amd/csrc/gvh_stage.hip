@@ -17,18 +17,32 @@ namespace gvh {
 // arrangements 0, 3 and 4; others run as 0.  With root64 (precision
 // GVH_MIXED_F64_ROOTS) the mixed form keeps the half-site root-finds in
 // FP64: arrangement 3 runs its lockstep in FP64, and arrangement 0's
-// one-at-a-time root-finds are FP64 in every form.
+// one-at-a-time root-finds are FP64 in every form.  Arrangement 5 is 3 in
+// blocks of four rows that stage the y stencil in LDS
+// (kt_flux_kernel_ystaged); it is built for the tabulated EoS with bulk in
+// FP64 and GVH_MIXED, needs a strip height that is a multiple of 4, and runs
+// as 3 otherwise.
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool MIXED>
 static hipError_t launch_z(const StageArgs &A, bool root64, hipStream_t st) {
   const long long ncell = (long long)A.nx * A.strip * A.nstrips * A.nz;
   const int block = BLOCK;
   const int grid = (int)(((ncell + block - 1) / block + 7) / 8 * 8);  // multiple of 8 (XCD order)
+  if constexpr (EOS == EOS_TABLE && BULK) {
+    if (A.variant == 5 && !root64 && A.strip % YROWS == 0 && A.ny >= YROWS) {
+      const long long nb = (ncell + YROWS * block - 1) / (YROWS * block);
+      const int grid4 = (int)((nb + 7) / 8 * 8);
+      using R = typename std::conditional<MIXED, float, double>::type;
+      hipLaunchKernelGGL((kt_flux_kernel_ystaged<EOS, BULK, STAGE2, HAS_Z, 1, R, R>),
+                         dim3(grid4), dim3(YROWS * block), 0, st, A);
+      return hipGetLastError();
+    }
+  }
   if constexpr (MIXED) {
     // FP32 stencil, limiter, flux and sources; FP32 pi/Pi storage
     if (root64 && A.variant != 0)
       hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 3, float, float>),
                          dim3(grid), dim3(block), 0, st, A);
-    else if (A.variant == 3)
+    else if (A.variant == 3 || A.variant == 5)
       hipLaunchKernelGGL((kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 1, float, float>),
                          dim3(grid), dim3(block), 0, st, A);
     else if (A.variant == 4)
@@ -51,6 +65,7 @@ static hipError_t launch_z(const StageArgs &A, bool root64, hipStream_t st) {
             dim3(grid), dim3(block), 0, st, A);
         break;
       case 3:
+      case 5:
         hipLaunchKernelGGL(
             (kt_flux_kernel<EOS, BULK, STAGE2, HAS_Z, true, true, 1, double, double>),
             dim3(grid), dim3(block), 0, st, A);

@@ -282,9 +282,15 @@ constexpr int BLOCK = GVH_BLOCK;
 // the four solves, then every variable once more for its fluxes.
 // Every stencil load is a uniform (scalar) base plus the thread's 32-bit
 // cell offset.
-template <int EOS, bool BULK, int DIR, bool KEEP_STATES, int LOCK, typename R, typename PS>
+// YSTAGE (y direction of the four-row blocks, kt_flux_kernel_ystaged): the
+// 45 stencil values of the nine inversion variables come from the block's
+// LDS copy of rows j-2 .. j+2 (ystage[variable][row slot][lane]) instead of
+// from memory.
+template <int EOS, bool BULK, int DIR, bool KEEP_STATES, int LOCK, typename R, typename PS,
+          bool YSTAGE = false>
 __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s, double ePrev,
-                                                 R v_s, double *acc) {
+                                                 R v_s, double *acc,
+                                                 const R *ystage = nullptr) {
   constexpr int NV = BULK ? 15 : 14;
   const long long st = DIR == 0 ? 1 : (DIR == 1 ? (long long)A.ncx : (long long)A.ncx * A.ncy);
   const long long G = A.G;
@@ -327,7 +333,12 @@ __device__ __forceinline__ void direction_update(const StageArgs &A, unsigned s,
       if (a == 8 && !BULK) continue;
       const int n = a < 8 ? a : 14;
 #pragma unroll
-      for (int m = 0; m < 5; ++m) raw[a][m] = ld(n, (m - 2) * st);
+      for (int m = 0; m < 5; ++m) {
+        if constexpr (YSTAGE && DIR == 1)
+          raw[a][m] = ystage[(a * 8 + (threadIdx.x >> 6) + m) * 64 + (threadIdx.x & 63)];
+        else
+          raw[a][m] = ld(n, (m - 2) * st);
+      }
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
@@ -576,9 +587,14 @@ __global__ __launch_bounds__(BLOCK) void kt_source_kernel(StageArgs A) {
 // copies.  acc starts from what kt_source_kernel stored at the cell.
 // FUSED: the source terms are computed here first (one kernel per stage);
 // otherwise kt_source_kernel has stored acc = q + dt S at the cell.
+// YSTAGE: the thread belongs to a four-row block that staged the y stencil
+// in LDS; every thread of the block reaches the barrier before the y
+// direction, and only threads of valid cells compute (the others only
+// staged rows for their neighbours).
 template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, bool FUSED, bool KEEP, int LOCK,
-          typename R, typename PS>
-__device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c, double *acc) {
+          typename R, typename PS, bool YSTAGE = false>
+__device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c, double *acc,
+                                           const R *ystage = nullptr) {
   constexpr int NV = BULK ? 15 : 14;
   const int i = c.i, j = c.j, k = c.k;
   const unsigned s = c.s;
@@ -591,49 +607,59 @@ __device__ __forceinline__ void stage_cell(const StageArgs &A, const CellPos &c,
                  : (double)ld_cell(reinterpret_cast<const PS *>(base + 4 * G) + (n - 4) * G, s);
   };
   const double *u = A.uin;
-  const double ut = ld_cell(u, s), ux = ld_cell(u + G, s), uy = ld_cell(u + 2 * G, s),
-               un = ld_cell(u + 3 * G, s);
-  const double e_s = ld_cell(A.ein, s);
-  if constexpr (FUSED) {
-    const double p_s = ld_cell(A.pin, s);
-    GradT<R> g;
-    const long long st[3] = {1, sy, sz};
+  double ut = 1.0, ux = 0.0, uy = 0.0, un = 0.0, e_s = 0.0;
+  if (!YSTAGE || c.valid) {
+    ut = ld_cell(u, s);
+    ux = ld_cell(u + G, s);
+    uy = ld_cell(u + 2 * G, s);
+    un = ld_cell(u + 3 * G, s);
+    e_s = ld_cell(A.ein, s);
+    if constexpr (FUSED) {
+      const double p_s = ld_cell(A.pin, s);
+      GradT<R> g;
+      const long long st[3] = {1, sy, sz};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (d == 2 && !HAS_Z) {
-        g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = R(0);
-        continue;
+      for (int d = 0; d < 3; ++d) {
+        if (d == 2 && !HAS_Z) {
+          g.ut[2] = g.ux[2] = g.uy[2] = g.un[2] = g.p[2] = R(0);
+          continue;
+        }
+        const R ih = R(A.inv_h[d]);
+        const double *um = u - st[d], *upl = u + st[d];
+        g.ut[d] = slope_t<R>(R(ld_cell(um, s)), R(ut), R(ld_cell(upl, s))) * ih;
+        g.ux[d] = slope_t<R>(R(ld_cell(um + G, s)), R(ux), R(ld_cell(upl + G, s))) * ih;
+        g.uy[d] = slope_t<R>(R(ld_cell(um + 2 * G, s)), R(uy), R(ld_cell(upl + 2 * G, s))) * ih;
+        g.un[d] = slope_t<R>(R(ld_cell(um + 3 * G, s)), R(un), R(ld_cell(upl + 3 * G, s))) * ih;
+        g.p[d] = slope_t<R>(R(ld_cell(A.pin - st[d], s)), R(p_s), R(ld_cell(A.pin + st[d], s))) * ih;
       }
-      const R ih = R(A.inv_h[d]);
-      const double *um = u - st[d], *upl = u + st[d];
-      g.ut[d] = slope_t<R>(R(ld_cell(um, s)), R(ut), R(ld_cell(upl, s))) * ih;
-      g.ux[d] = slope_t<R>(R(ld_cell(um + G, s)), R(ux), R(ld_cell(upl + G, s))) * ih;
-      g.uy[d] = slope_t<R>(R(ld_cell(um + 2 * G, s)), R(uy), R(ld_cell(upl + 2 * G, s))) * ih;
-      g.un[d] = slope_t<R>(R(ld_cell(um + 3 * G, s)), R(un), R(ld_cell(upl + 3 * G, s))) * ih;
-      g.p[d] = slope_t<R>(R(ld_cell(A.pin - st[d], s)), R(p_s), R(ld_cell(A.pin + st[d], s))) * ih;
+      double Qd[15];
+      R Q[15], S[15];
+#pragma unroll
+      for (int n = 0; n < 15; ++n) {
+        Qd[n] = (n < NV) ? q_at(A.qin, n) : 0.0;
+        Q[n] = R(Qd[n]);
+      }
+      S[14] = R(0);
+      const double *up = A.uprev;
+      source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un),
+                                 R(ld_cell(up, s)), R(ld_cell(up + G, s)),
+                                 R(ld_cell(up + 2 * G, s)), R(ld_cell(up + 3 * G, s)), g, S);
+#pragma unroll
+      for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Qd[n] + A.dt * (double)S[n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) acc[n * BLOCK] = ld_cell(A.qout + n * G, s);
     }
-    double Qd[15];
-    R Q[15], S[15];
-#pragma unroll
-    for (int n = 0; n < 15; ++n) {
-      Qd[n] = (n < NV) ? q_at(A.qin, n) : 0.0;
-      Q[n] = R(Qd[n]);
-    }
-    S[14] = R(0);
-    const double *up = A.uprev;
-    source_terms<EOS, BULK, R>(A, Q, e_s, R(e_s), R(p_s), R(ut), R(ux), R(uy), R(un),
-                               R(ld_cell(up, s)), R(ld_cell(up + G, s)),
-                               R(ld_cell(up + 2 * G, s)), R(ld_cell(up + 3 * G, s)), g, S);
-#pragma unroll
-    for (int n = 0; n < 15; ++n) acc[n * BLOCK] = Qd[n] + A.dt * (double)S[n];
-  } else {
-#pragma unroll
-    for (int n = 0; n < NV; ++n) acc[n * BLOCK] = ld_cell(A.qout + n * G, s);
-  }
 
-  // flux velocity of the cell itself for the directional sources (v = u/u^tau)
-  direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(ux), R(ut)), acc);
-  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(uy), R(ut)), acc);
+    // flux velocity of the cell itself for the directional sources (v = u/u^tau)
+    direction_update<EOS, BULK, 0, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(ux), R(ut)), acc);
+  }
+  if constexpr (YSTAGE) {
+    __syncthreads();
+    if (!c.valid) return;
+  }
+  direction_update<EOS, BULK, 1, KEEP, LOCK, R, PS, YSTAGE>(A, s, e_s, div_st(R(uy), R(ut)), acc,
+                                                            ystage);
   if constexpr (HAS_Z) direction_update<EOS, BULK, 2, KEEP, LOCK, R, PS>(A, s, e_s, div_st(R(un), R(ut)), acc);
 
   if constexpr (STAGE2) {
@@ -727,6 +753,80 @@ __global__ __launch_bounds__(BLOCK, (stage_waves_per_simd<R, LOCK>())) void kt_f
   const CellPos c = cell_of_thread(A);
   if (!c.valid) return;
   stage_cell<EOS, BULK, STAGE2, HAS_Z, FUSED, KEEP, LOCK, R, PS>(A, c, acc_lds + threadIdx.x);
+}
+
+// ------------------------------------- y stencil staged in LDS (arrangement 5)
+// Blocks of four waves on four neighbouring rows of a strip: lane L of every
+// wave has the same x and row group g, and wave w takes row 4 g + w (the
+// strip height is a multiple of 4).  For the nine inversion variables the
+// block loads rows j0 - 2 .. j0 + 5 of its lanes once into LDS -- each wave
+// its own row and one halo row -- so a cell's y stencil (rows j - 2 .. j + 2)
+// is read from LDS instead of five times from memory.  Rows past the
+// ghosted lattice are not loaded; only cells past the lattice would read them.
+constexpr int YROWS = 4;
+
+struct CellRows {
+  CellPos c;
+  int jg;  // ghosted row of the thread's wave, not clamped to the lattice
+};
+
+__device__ __forceinline__ CellRows cell_of_block_rows(const StageArgs &A, unsigned b,
+                                                       unsigned nb) {
+  const unsigned lb = (b & 7u) * (nb >> 3) + (b >> 3);
+  const unsigned by = (unsigned)A.strip, nx = (unsigned)A.nx, nz = (unsigned)A.nz;
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const unsigned T = lb * 64u + lane;
+  const unsigned r = T / nx;
+  const int i = (int)(T - r * nx);
+  const unsigned gpr = by / YROWS;  // row groups per strip
+  const unsigned r2 = r / gpr;
+  const int jj = (int)(r - r2 * gpr) * YROWS + (int)w;
+  const unsigned yb_ = r2 / nz;
+  const int k = (int)(r2 - yb_ * nz);
+  const int yb = (int)yb_;
+  const int j = yb * (int)by + jj;
+  CellRows o;
+  o.c.valid = j < A.ny && yb < A.nstrips;
+  o.c.i = i + NG;
+  o.c.j = (o.c.valid ? j : 0) + NG;
+  o.c.k = k + NG;
+  o.c.s = (unsigned)o.c.i + (unsigned)A.ncx * ((unsigned)o.c.j + (unsigned)A.ncy * (unsigned)o.c.k);
+  o.jg = j + NG;
+  return o;
+}
+
+template <int EOS, bool BULK, bool STAGE2, bool HAS_Z, int LOCK, typename R, typename PS>
+__global__ __launch_bounds__(YROWS * BLOCK, (stage_waves_per_simd<R, LOCK>())) void
+kt_flux_kernel_ystaged(StageArgs A) {
+  __shared__ double acc_lds[YROWS][15 * BLOCK];
+  __shared__ R ystage[9 * 8 * BLOCK];
+  const CellRows cr = cell_of_block_rows(A, blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // own row -> slot w + 2; halo: waves 0, 1 rows j - 2 (slots 0, 1), waves
+  // 2, 3 rows j + 2 (slots 6, 7)
+  const int hrow = w < 2 ? cr.jg - 2 : cr.jg + 2;
+  const int hslot = w < 2 ? w : w + 4;
+  const long long G = A.G;
+  const double *qT = A.qin;
+  const PS *qP = reinterpret_cast<const PS *>(A.qin + 4 * G);
+  const unsigned ncx = (unsigned)A.ncx, ncy = (unsigned)A.ncy;
+  const unsigned colbase = (unsigned)cr.c.i + ncx * ncy * (unsigned)cr.c.k;
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const int row = pass == 0 ? cr.jg : hrow;
+    const int slot = pass == 0 ? w + 2 : hslot;
+    const bool in = row >= 0 && row < A.ncy;
+    const unsigned sr = in ? colbase + ncx * (unsigned)row : cr.c.s;
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      if (a == 8 && !BULK) continue;
+      const int n = a < 8 ? a : 14;
+      const R v = n < 4 ? R(ld_cell(qT + n * G, sr)) : R(ld_cell(qP + (n - 4) * G, sr));
+      ystage[(a * 8 + slot) * BLOCK + lane] = in ? v : R(0);
+    }
+  }
+  stage_cell<EOS, BULK, STAGE2, HAS_Z, true, true, LOCK, R, PS, true>(A, cr.c, acc_lds[w] + lane,
+                                                                     ystage);
 }
 
 }  // namespace gvh

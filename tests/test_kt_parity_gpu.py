@@ -80,9 +80,12 @@ def orc_e_reg(case):
     return O.lib().kto_e_of_T(case.oracle().P, case.T_reg_GeV / gpuvh.HBARC)
 
 
-def test_2d_table_eos_with_flow_and_pi():
+@pytest.mark.parametrize("variant", [3, 5])
+def test_2d_table_eos_with_flow_and_pi(variant, monkeypatch):
     """2+1D, tabulated EoS, bulk, flowing state with initial pi: 31x27x1,
-    30 steps (ragged nx != ny)."""
+    30 steps (ragged nx != ny); the default arrangement and the four-row
+    blocks with the y stencil in LDS."""
+    monkeypatch.setenv("GVH_VARIANT", str(variant))
     case = Case(31, 27, 1, 0.1, 0.1, 0.01, 0.5, 0.16, bulk=True, eos=gpuvh.EOS_TABLE,
                 zeta_tpeak=0.16 / gpuvh.HBARC)
     state = hotspot_state(case, seed=3, n_hot=10, amp=6.0, pi_noise=0.02, u_noise=0.1)
@@ -136,7 +139,8 @@ def test_field_roundtrip_and_errors():
         ctx.get(99)
 
 
-@pytest.mark.parametrize("variant,strip", [(0, 16), (1, 4), (2, 8), (0, 1), (2, 100), (3, 16), (3, 5), (4, 16)])
+@pytest.mark.parametrize("variant,strip", [(0, 16), (1, 4), (2, 8), (0, 1), (2, 100), (3, 16), (3, 5),
+                                           (4, 16), (5, 16), (5, 8), (5, 4), (5, 5)])
 def test_kernel_arrangements_agree(monkeypatch, variant, strip):
     """Every stage-kernel arrangement and traversal strip gives the oracle's
     result (23x21x9, tabulated EoS, shear+bulk, 8 steps)."""

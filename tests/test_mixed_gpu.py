@@ -65,7 +65,7 @@ def oracle_fields(name, case, state, nsteps, fields):
 
 @pytest.mark.parametrize("precision", [gpuvh.MIXED, gpuvh.MIXED_F64_ROOTS],
                          ids=["mixed", "mixed_f64_roots"])
-@pytest.mark.parametrize("variant", [0, 3])
+@pytest.mark.parametrize("variant", [0, 3, 5])
 @pytest.mark.parametrize("name,case,nsteps", CASES, ids=[c[0] for c in CASES])
 def test_mixed_against_oracle(name, case, nsteps, variant, precision, monkeypatch):
     """After nsteps the mixed forms' e and u^mu agree with the oracle within
