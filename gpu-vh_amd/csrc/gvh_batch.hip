@@ -13,9 +13,11 @@
 // Launch groups are events of the same dimensionality and stage-kernel
 // arrangement.  The sequence is captured once per batch composition into
 // two HIP graphs (record and check; stages and advance) and replayed each
-// step.  The freeze-out windows due at a step are searched on other streams
-// once the first graph has run, beside the stages (the searches read only
-// the history ring).  Between steps the host retires finished events and
+// step.  The freeze-out window due at a step is searched on another stream
+// once the first graph has run, beside the stages and, phase by phase, the
+// following steps: an event's ring has two halves, alternating by window, so
+// the steps after a window record into the half the search does not read.
+// Between steps the host retires finished events and
 // admits queued ones while their HBM footprint (state, history ring, surface
 // buffers) fits the budget: requests that arrive while others run join at
 // the next step.  An event's cells are cut into tiles exactly as a launch
@@ -146,7 +148,7 @@ struct gvh_batch {
   // before the event's next window is searched, and before the event retires.
   struct Search {
     int phase = 0;
-    int n = 0, m = 0;   // loop index and window of the search
+    int m = 0;          // window of the search (loop index / fofreq)
     double t = 0;       // the loop's time at that window
     long long ne = 0;   // rows
     hipEvent_t ev = nullptr;  // recorded after the phase's device-to-host copy
@@ -266,7 +268,6 @@ int start_search(gvh_batch *b, int s, int n) {
   if ((rc = complete_search(b, s))) return rc;
   if (!W.ev) HIP_TRY(hipEventCreateWithFlags(&W.ev, hipEventDisableTiming));
   const hipStream_t st = search_stream(b, s);
-  W.n = n;
   W.m = n / b->fofreq;
   W.t = L.ctx->Hp.initialProperTimePoint + L.steps * L.ctx->L.latticeSpacingProperTime;
   W.ne = 0;

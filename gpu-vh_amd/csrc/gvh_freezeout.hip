@@ -681,8 +681,9 @@ extern "C" int gvh_freezeout_record(gvh_context *c, int n) {
 
 // The window search in three phases, each ending in a small device-to-host
 // copy that sizes the next: one context's search runs them back to back
-// (gvh_freezeout_find); a batch runs each phase for all events whose window
-// is due, on several streams, with one wait per phase.
+// (gvh_freezeout_find); a batch issues one phase per step for each event
+// whose search is in flight, on several streams, once the previous phase
+// has finished.
 namespace gvh {
 
 // Phase 1: surface cubes per (it, ix, iy) row and their scan; the number of
